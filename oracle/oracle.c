@@ -1,0 +1,597 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * Plain-C CPU restatement of the reference's primordial-chemistry ROS2S grid
+ * advance.  It is the checker for the HIP path: only tests/, the smoke check
+ * and bench.py's CPU-baseline leg load it.  The product never links it.
+ *
+ * Every function states the reference code it follows (reproducer.cpp:LINE).
+ * The network right-hand side and Jacobian are the generated straight-line
+ * code emitted from the committed DAG IR (_build/chem_network.h).
+ *
+ * Pinning (tests/test_oracle.py): each function below is compared bit for bit
+ * with the reference's own compiled CPU code (oracle/_ref/libwr_ref.so, built
+ * from reproducer.cpp by oracle/Makefile) and with committed golden vectors
+ * written from it (tests/golden/): RHS/J at several hundred states spanning
+ * every temperature branch, LU on the reference's LU test matrices and on real
+ * E matrices, single ROS2S integrations with all seven counters, prepare and
+ * advance semantics, and the grid-1 known answer (438 global steps).  Closed
+ * forms pin the rest: SplitMix64 against its published test vector, the
+ * free-fall time formula, mass/charge conservation of the RHS, LU against
+ * brute-force permutation checks.  No function is "parity unpinned".
+ *
+ * Build: oracle/Makefile, gcc -O2 -ffp-contract=off (no FMA contraction; x86-64
+ * SSE2 arithmetic is IEEE double).
+ */
+#include <math.h>
+#include <float.h>
+#include <stdint.h>
+#include <string.h>
+
+#define NSPEC 14
+#define NEQ 15
+
+typedef struct {
+    double rho, T, e;
+    double xn[NSPEC];
+} or_burn_t;
+
+typedef struct {
+    uint64_t internal_steps, rhs_calls, jacobian_calls, decompositions,
+        linear_solves, accepted_steps, rejected_steps;
+} or_stats;
+
+/* Same 216-byte layout as CollapseState (reproducer.cpp:7455-7461). */
+typedef struct {
+    or_burn_t current;
+    double time;
+    double density_driver;
+    int32_t completed_steps;
+    int32_t pad_;
+    or_stats stats;
+} or_cell;
+
+_Static_assert(sizeof(or_cell) == 216, "or_cell must match CollapseState");
+
+enum { OR_SUCCESS = 1, OR_BAD_INPUTS = -1, OR_DT_UNDERFLOW = -2, OR_TOO_MANY_STEPS = -4,
+       OR_TOO_MUCH_ACCURACY = -5, OR_LU_ERROR = -7 };
+
+/* ---- generated network ------------------------------------------------- */
+#define CHEM_FN static inline
+#define CHEM_SINK_PARAMS , double *jsink
+#define CHEM_JAC_STORE(r, c, v) (jsink[(r) * NEQ + (c)] = (v))
+#include "_build/chem_network.h"
+
+/* ---- network constants and EOS (reproducer.cpp:199-339) ----------------- */
+static const double k_mass[NSPEC] = {
+    9.10938188e-28, 1.67262158e-24, 1.67353251819e-24, 1.67444345638e-24,
+    3.34512158e-24, 3.34603251819e-24, 3.34615409819e-24, 3.34694345638e-24,
+    3.34706503638e-24, 5.01865409819e-24, 5.01956503638e-24, 6.69024316e-24,
+    6.69115409819e-24, 6.69206503638e-24};
+static const double k_nA = 6.02214076e+23, k_kB = 1.3806490000000002e-16,
+                    k_mp = 1.67262192595e-24;
+static const double k_floor = 1.0e-100;
+static const double k_G = 6.674e-8;
+static const double k_pi = 3.141592653589793238462643383279502884;
+
+static double gamma_of(int n) { return (n == 6 || n == 8 || n == 9 || n == 10) ? 1.4 : 5.0 / 3.0; }
+
+/* density, reproducer.cpp:270-276 */
+static double density_of(const double *xn) {
+    double rho = 0.0;
+    for (int n = 0; n < NSPEC; ++n) rho += xn[n] * k_mass[n];
+    return rho;
+}
+
+/* eos_sums_from_number_densities, reproducer.cpp:284-304: returns the
+ * denominator product used by eos_re/eos_rt pieces. */
+static void eos_sums(const double *xn, double *abarinv, double *gammasinv) {
+    double rhotot = 0.0, sa = 0.0, sg = 0.0;
+    for (int n = 0; n < NSPEC; ++n) rhotot += xn[n] * k_mass[n];
+    for (int n = 0; n < NSPEC; ++n) {
+        sa += xn[n];
+        sg += (xn[n] * k_mp / rhotot) * (1.0 / (gamma_of(n) - 1.0));
+    }
+    sa *= k_mp / rhotot;
+    sg /= sa;
+    *abarinv = sa;
+    *gammasinv = sg;
+}
+
+/* eos_rt, reproducer.cpp:306-309 */
+static void eos_rt(or_burn_t *s) {
+    double sa, sg;
+    eos_sums(s->xn, &sa, &sg);
+    s->e = sg * sa * (k_nA * k_kB) * s->T;
+}
+
+/* eos_re, reproducer.cpp:311-314 */
+static void eos_re(or_burn_t *s) {
+    double sa, sg;
+    eos_sums(s->xn, &sa, &sg);
+    s->T = s->e / (sg * (k_nA * k_kB) * sa);
+}
+
+/* balance_charge, reproducer.cpp:316-319 */
+static void balance_charge(or_burn_t *s) {
+    const double *x = s->xn;
+    s->xn[0] = -x[3] - x[7] + x[1] + x[12] + x[6] + x[4] + x[9] + 2.0 * x[11];
+}
+
+/* normalize_number_densities_to_density, reproducer.cpp:321-332 */
+static void normalize(or_burn_t *s) {
+    double mf[NSPEC], sum = 0.0;
+    for (int n = 0; n < NSPEC; ++n) {
+        mf[n] = k_mass[n] * s->xn[n] / s->rho;
+        sum += mf[n];
+    }
+    for (int n = 0; n < NSPEC; ++n) {
+        mf[n] /= sum;
+        s->xn[n] = mf[n] * s->rho / k_mass[n];
+    }
+}
+
+/* std::max(a, b) as the reference evaluates it: (a < b) ? b : a */
+static double ref_max(double a, double b) { return (a < b) ? b : a; }
+static double ref_min(double a, double b) { return (b < a) ? b : a; }
+
+/* floor_and_normalize_number_densities, reproducer.cpp:334-339 */
+static void floor_and_normalize(or_burn_t *s) {
+    for (int n = 0; n < NSPEC; ++n) s->xn[n] = ref_max(s->xn[n], k_floor);
+    normalize(s);
+}
+
+/* burn_state_from_y, reproducer.cpp:7029-7038: X = max(y, floor), T from e. */
+static double temperature_from_y(const double *y, double *X) {
+    or_burn_t b;
+    for (int n = 0; n < NSPEC; ++n) {
+        b.xn[n] = ref_max(y[n], k_floor);
+        X[n] = b.xn[n];
+    }
+    b.e = y[NSPEC];
+    eos_re(&b);
+    return b.T;
+}
+
+/* PrimordialChem::rhs, reproducer.cpp:7040-7050 */
+void or_rhs(const double *y, double *dydt) {
+    double X[NSPEC];
+    const double T = temperature_from_y(y, X);
+    chem_rhs(T, X, dydt);
+}
+
+/* PrimordialChem::jacobian, reproducer.cpp:7052-7060 (row-major 15x15) */
+void or_jac(const double *y, double *jac) {
+    double X[NSPEC], f[NEQ];
+    const double T = temperature_from_y(y, X);
+    chem_fused(T, X, f, jac);
+}
+
+/* J and f(y) at one state, as the fused evaluation the GPU path uses. */
+void or_jac_rhs(const double *y, double *jac, double *dydt) {
+    double X[NSPEC];
+    const double T = temperature_from_y(y, X);
+    chem_fused(T, X, dydt, jac);
+}
+
+double or_temperature_from_y(const double *y) {
+    double X[NSPEC];
+    return temperature_from_y(y, X);
+}
+
+/* ---- dense LU, LINPACK conventions (reproducer.cpp:96-181) ------------- */
+
+/* lu_decomposition<15>: first strict maximum of |a_ik| is the pivot, rows are
+ * exchanged from column k on (earlier multipliers stay in place), multipliers
+ * are stored negated; info = k+1 for a zero pivot, N if the last is zero. */
+int or_lu(double *a, int *ipvt) {
+    int info = 0;
+    for (int k = 0; k < NEQ - 1; ++k) {
+        int p = k;
+        double best = fabs(a[k * NEQ + k]);
+        for (int i = k + 1; i < NEQ; ++i) {
+            const double v = fabs(a[i * NEQ + k]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        ipvt[k] = p;
+        if (a[p * NEQ + k] == 0.0) {
+            info = k + 1;
+            continue;
+        }
+        if (p != k) {
+            for (int j = k; j < NEQ; ++j) {
+                const double t = a[p * NEQ + j];
+                a[p * NEQ + j] = a[k * NEQ + j];
+                a[k * NEQ + j] = t;
+            }
+        }
+        const double m = -1.0 / a[k * NEQ + k];
+        for (int i = k + 1; i < NEQ; ++i) a[i * NEQ + k] *= m;
+        for (int j = k + 1; j < NEQ; ++j) {
+            const double t = a[k * NEQ + j];
+            for (int i = k + 1; i < NEQ; ++i) a[i * NEQ + j] += t * a[i * NEQ + k];
+        }
+    }
+    ipvt[NEQ - 1] = NEQ - 1;
+    if (a[(NEQ - 1) * NEQ + NEQ - 1] == 0.0) info = NEQ;
+    return info;
+}
+
+/* lu_solve<15>: interchanges and L applied in elimination order, then U. */
+void or_lu_solve(const double *lu, const int *ipvt, double *x) {
+    for (int k = 0; k < NEQ - 1; ++k) {
+        const int p = ipvt[k];
+        const double t = x[p];
+        if (p != k) {
+            x[p] = x[k];
+            x[k] = t;
+        }
+        for (int j = k + 1; j < NEQ; ++j) x[j] += t * lu[j * NEQ + k];
+    }
+    for (int k = NEQ - 1; k >= 0; --k) {
+        x[k] /= lu[k * NEQ + k];
+        const double t = -x[k];
+        for (int j = 0; j < k; ++j) x[j] += t * lu[j * NEQ + k];
+    }
+}
+
+/* ---- ROS2S (reproducer.cpp:7066-7347) ---------------------------------- */
+static const double c_gamma = 0.292893218813452, c_ct2 = 0.585786437626905,
+                    c_a21 = 2.0000000000000036, c_a31 = 6.828427124746214,
+                    c_a32 = 3.4142135623731007, c_c21 = -6.828427124746214,
+                    c_c31 = -10.949747468305889, c_c32 = -7.535533905932761,
+                    c_b1 = 6.828427124746214, c_b2 = 3.414213562373101, c_b3 = 1.0,
+                    c_e1 = -0.23570226039551292, c_e2 = -0.23570226039551567,
+                    c_e3 = -0.13807118745769906;
+static const double k_uround = 1.0e-16, k_fac_min = 0.2, k_fac_max = 6.0, k_safe = 0.9;
+
+/* Tolerances of configure_ros2s (reproducer.cpp:7409-7412, 7582-7590). */
+static double rtol_of(int i) { return i < NSPEC ? 1.0e-4 : 1.0e-6; }
+static double atol_of(int i) { return i < NSPEC ? 1.0e-4 : 1.0e-6; }
+
+typedef struct {
+    int n_step, n_rhs, n_jac, n_decomp, n_solve, n_accept, n_reject;
+} or_counters;
+
+/* RODAS<PrimordialChem>::integrate from t=0 to tout; y updated in place.
+ * The checks on safe/fac/tolerances (7208-7218) cannot fail for the
+ * collapse configuration and are kept for completeness. */
+int or_integrate(double *y, double tout, double dt0, int max_steps, int *counters,
+                 double *t_end, double *dt_end) {
+    or_counters c = {0, 0, 0, 0, 0, 0, 0};
+    double fjac[NEQ * NEQ], e[NEQ * NEQ];
+    double ynew[NEQ], ak1[NEQ], ak2[NEQ], work[NEQ], dy[NEQ];
+    int ip[NEQ];
+    double t = 0.0;
+    int result;
+    if (max_steps <= 0) max_steps = 10000000;
+
+    if (tout <= t) {
+        result = OR_BAD_INPUTS;
+        *t_end = t;
+        *dt_end = dt0;
+        goto done;
+    }
+    for (int i = 0; i < NEQ; ++i) {
+        if (atol_of(i) <= 0.0 || rtol_of(i) <= 10.0 * k_uround) {
+            result = OR_TOO_MUCH_ACCURACY;
+            *t_end = t;
+            *dt_end = dt0;
+            goto done;
+        }
+    }
+    {
+        const double hmaxn = tout - t;
+        double h = dt0;
+        if (fabs(h) <= 10.0 * k_uround) h = 1.0e-6;
+        h = ref_min(fabs(h), hmaxn);
+        int reject = 0, last = 0, nsing = 0, n_step = 0, n_accept = 0;
+        double hacc = 0.0, erracc = 1.0, hopt = h, x = t;
+        for (;;) {
+            if (n_step > max_steps) {
+                *t_end = x; *dt_end = h; result = OR_TOO_MANY_STEPS; goto done;
+            }
+            if (0.1 * fabs(h) <= fabs(x) * k_uround) {
+                *t_end = x; *dt_end = h; result = OR_DT_UNDERFLOW; goto done;
+            }
+            if (last) {
+                *t_end = x; *dt_end = hopt; result = OR_SUCCESS; goto done;
+            }
+            hopt = h;
+            if (x + h * 1.0001 >= tout) {
+                h = tout - x;
+                last = 1;
+            }
+            or_jac(y, fjac);
+            c.n_jac += 1;
+            for (;;) {
+                const double fac = 1.0 / (h * c_gamma);
+                for (int i = 0; i < NEQ; ++i) {
+                    for (int j = 0; j < NEQ; ++j) e[i * NEQ + j] = -fjac[i * NEQ + j];
+                    e[i * NEQ + i] += fac;
+                }
+                if (or_lu(e, ip) != 0) {
+                    nsing += 1;
+                    if (nsing >= 5) {
+                        *t_end = x; *dt_end = h; result = OR_LU_ERROR; goto done;
+                    }
+                    h *= 0.5;
+                    reject = 1;
+                    last = 0;
+                    continue;
+                }
+                c.n_decomp += 1;
+
+                or_rhs(y, ak1);
+                or_lu_solve(e, ip, ak1);
+                c.n_solve += 1;
+                for (int i = 0; i < NEQ; ++i) {
+                    ynew[i] = y[i] + c_a21 * ak1[i];
+                    ak2[i] = (c_c21 / h) * ak1[i];
+                }
+                or_rhs(ynew, dy);
+                for (int i = 0; i < NEQ; ++i) ak2[i] += dy[i];
+                or_lu_solve(e, ip, ak2);
+                c.n_solve += 1;
+                for (int i = 0; i < NEQ; ++i) {
+                    ynew[i] = y[i] + c_a31 * ak1[i] + c_a32 * ak2[i];
+                    work[i] = (c_c31 * ak1[i] + c_c32 * ak2[i]) / h;
+                }
+                or_rhs(ynew, dy);
+                for (int i = 0; i < NEQ; ++i) work[i] += dy[i];
+                or_lu_solve(e, ip, work);
+                c.n_solve += 1;
+                for (int i = 0; i < NEQ; ++i) {
+                    const double ak3 = work[i];
+                    ynew[i] = y[i] + c_b1 * ak1[i] + c_b2 * ak2[i] + c_b3 * ak3;
+                    work[i] = c_e1 * ak1[i] + c_e2 * ak2[i] + c_e3 * ak3;
+                }
+                c.n_rhs += 3;
+                n_step += 1;
+                c.n_step = n_step;
+
+                /* error_norm, reproducer.cpp:7194-7202 */
+                double err = 0.0;
+                for (int i = 0; i < NEQ; ++i) {
+                    const double sk = atol_of(i) + rtol_of(i) * ref_max(fabs(y[i]), fabs(ynew[i]));
+                    const double term = work[i] / sk;
+                    err += term * term;
+                }
+                err = sqrt(err / (double)NEQ);
+
+                const double raw_fac = cbrt(err) / k_safe;
+                const double fac_step = ref_max(1.0 / k_fac_max, ref_min(1.0 / k_fac_min, raw_fac));
+                double hnew = h / fac_step;
+                if (err <= 1.0) {
+                    n_accept += 1;
+                    c.n_accept = n_accept;
+                    if (n_accept > 1) {
+                        const double facgus = ref_max(
+                            1.0 / k_fac_max,
+                            ref_min(1.0 / k_fac_min,
+                                    (hacc / h) * cbrt((err * err) / erracc) / k_safe));
+                        hnew = h / ref_max(fac_step, facgus);
+                    }
+                    hacc = h;
+                    erracc = ref_max(1.0e-2, err);
+                    memcpy(y, ynew, sizeof(ynew));
+                    x += h;
+                    if (fabs(hnew) > hmaxn) hnew = hmaxn;
+                    if (reject) hnew = ref_min(fabs(hnew), fabs(h));
+                    reject = 0;
+                    h = hnew;
+                    break;
+                }
+                reject = 1;
+                last = 0;
+                h = hnew;
+                if (n_accept >= 1) c.n_reject += 1;
+            }
+        }
+    }
+done:
+    counters[0] = c.n_step;
+    counters[1] = c.n_rhs;
+    counters[2] = c.n_jac;
+    counters[3] = c.n_decomp;
+    counters[4] = c.n_solve;
+    counters[5] = c.n_accept;
+    counters[6] = c.n_reject;
+    return result;
+}
+
+/* ---- cell driver (reproducer.cpp:7406-7785) ----------------------------- */
+
+static uint64_t clamp_u64(int v) { return v > 0 ? (uint64_t)v : 0u; }
+
+/* burn_ros2s, reproducer.cpp:7592-7627 */
+int or_burn_ros2s(or_burn_t *s, double dt, or_stats *st) {
+    double y[NEQ], t_end, dt_end;
+    int cnt[7];
+    eos_rt(s);
+    for (int n = 0; n < NSPEC; ++n) y[n] = s->xn[n];
+    y[NSPEC] = s->e;
+    const int r = or_integrate(y, dt, dt, 10000000, cnt, &t_end, &dt_end);
+    st->internal_steps += clamp_u64(cnt[0]);
+    st->rhs_calls += clamp_u64(cnt[1]);
+    st->jacobian_calls += clamp_u64(cnt[2]);
+    st->decompositions += clamp_u64(cnt[3]);
+    st->linear_solves += clamp_u64(cnt[4]);
+    st->accepted_steps += clamp_u64(cnt[5]);
+    st->rejected_steps += clamp_u64(cnt[6]);
+    if (r != OR_SUCCESS) return r;
+    for (int n = 0; n < NSPEC; ++n) s->xn[n] = y[n];
+    s->e = y[NSPEC];
+    return r;
+}
+
+int or_burn(double *burn, double dt, uint64_t *stats) {
+    return or_burn_ros2s((or_burn_t *)burn, dt, (or_stats *)stats);
+}
+
+/* make_initial_state / make_collapse_state, reproducer.cpp:7424-7426, 7533-7546 */
+void or_make_collapse_state(or_cell *c) {
+    static const double n0[NSPEC] = {1.0e-4, 1.0e-4, 1.0e0, 1.0e-40, 1.0e-40, 1.0e-40, 1.0e-40,
+                                     1.0e-40, 1.0e-6, 1.0e-40, 1.0e-40, 1.0e-40, 1.0e-40, 0.0775};
+    memset(c, 0, sizeof(*c));
+    c->current.T = 1.0e2;
+    memcpy(c->current.xn, n0, sizeof(n0));
+    c->current.rho = density_of(c->current.xn);
+    normalize(&c->current);
+    eos_rt(&c->current);
+    c->density_driver = c->current.rho;
+}
+
+/* splitmix64 (reproducer.cpp:7548-7553; Steele, Lea & Flood 2014). */
+uint64_t or_splitmix64(uint64_t v) {
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+/* perturbation_factor, reproducer.cpp:7555-7562 */
+double or_perturbation_factor(int cell, int step) {
+    const uint64_t seed = ((uint64_t)(uint32_t)cell << 32) ^ ((uint64_t)(uint32_t)step << 16);
+    const uint64_t bits = or_splitmix64(seed) >> 11;
+    const double unit = (double)bits * (1.0 / (double)(1ULL << 53));
+    return 1.0 + 0.2 * (2.0 * unit - 1.0);
+}
+
+/* apply_perturbation, reproducer.cpp:7564-7580 */
+void or_apply_perturbation(or_cell *c, int cell, int step, int enabled) {
+    if (!enabled || step == 0 || step % 20 != 0) return;
+    const double f = or_perturbation_factor(cell, step);
+    c->density_driver *= f;
+    c->current.rho *= f;
+    for (int n = 0; n < NSPEC; ++n) c->current.xn[n] *= f;
+    floor_and_normalize(&c->current);
+    balance_charge(&c->current);
+    floor_and_normalize(&c->current);
+    eos_re(&c->current);
+}
+
+static int valid_positive(double v) { return v > 0.0 && isfinite(v); }
+
+/* collapse_timestep, reproducer.cpp:7629-7644 */
+double or_collapse_timestep(const or_cell *c) {
+    const double rho = density_of(c->current.xn);
+    if (!valid_positive(rho)) return -1.0;
+    const double tff = sqrt(k_pi * 3.0 / (32.0 * rho * k_G));
+    if (!valid_positive(tff)) return -1.0;
+    const double dt = 1.0e-1 * tff;
+    return valid_positive(dt) ? dt : -1.0;
+}
+
+/* post-step sequence, reproducer.cpp:7773-7776 */
+void or_post_step(double *burn) {
+    or_burn_t *b = (or_burn_t *)burn;
+    floor_and_normalize(b);
+    balance_charge(b);
+    floor_and_normalize(b);
+    eos_re(b);
+}
+
+/* prepare_grid_timestep_kernel body, reproducer.cpp:7706-7733 */
+double or_prepare_cell(or_cell *s, int cell, int completed, double grid_time, int step,
+                       int perturb, int *failure) {
+    const double none = DBL_MAX;
+    if (s->completed_steps < completed) return none;
+    if (s->completed_steps != completed || s->time != grid_time) {
+        *failure = OR_BAD_INPUTS;
+        return none;
+    }
+    or_apply_perturbation(s, cell, step, perturb);
+    const double dt = or_collapse_timestep(s);
+    if (!valid_positive(dt) || !valid_positive(s->density_driver)) {
+        *failure = OR_BAD_INPUTS;
+        return none;
+    }
+    if (dt < 10.0) {
+        const double tff = dt / 1.0e-1;
+        s->density_driver += dt * (s->density_driver / tff);
+        return none;
+    }
+    return dt;
+}
+
+/* advance_collapse_gridwide_kernel body, reproducer.cpp:7745-7784 */
+int or_advance_cell(or_cell *s, int completed, double next_grid_time, double dt_grid,
+                    int *failure) {
+    if (s->completed_steps != completed) return 0;
+    const double local_dt = or_collapse_timestep(s);
+    if (local_dt < 10.0) return 0;
+    const double old_density = s->density_driver;
+    const double tff = local_dt / 1.0e-1;
+    s->density_driver += dt_grid * (s->density_driver / tff);
+    if (!valid_positive(s->density_driver) || s->density_driver > 2.0e-6) return 0;
+    const double ratio = s->density_driver / old_density;
+    for (int n = 0; n < NSPEC; ++n) s->current.xn[n] *= ratio;
+    s->current.rho *= ratio;
+    const int r = or_burn_ros2s(&s->current, dt_grid, &s->stats);
+    if (r != OR_SUCCESS) {
+        *failure = r;
+        return 0;
+    }
+    or_post_step((double *)&s->current);
+    s->time = next_grid_time;
+    s->completed_steps = completed + 1;
+    return 1;
+}
+
+/* Host loop of run_cells_cuda, reproducer.cpp:7859-7959 (gridwide-independent
+ * policy), cells in parallel inside each phase.  max_steps<=0 means 1000. */
+int or_run_gridwide(or_cell *cells, int n, int perturb, int max_steps, int *completed_out) {
+    const int limit = max_steps > 0 ? max_steps : 1000;
+    double grid_time = 0.0;
+    int completed = *completed_out;
+    int result = OR_SUCCESS;
+    for (int step = 0; result == OR_SUCCESS && step < limit; ++step) {
+        int failure = OR_SUCCESS;
+        double dt_grid = DBL_MAX;
+#pragma omp parallel for schedule(dynamic, 16) reduction(min : dt_grid)
+        for (int c = 0; c < n; ++c) {
+            int f = OR_SUCCESS;
+            const double cand = or_prepare_cell(&cells[c], c, completed, grid_time, step, perturb, &f);
+            if (cand < dt_grid) dt_grid = cand;
+            if (f != OR_SUCCESS) {
+#pragma omp critical
+                if (failure == OR_SUCCESS) failure = f;
+            }
+        }
+        if (failure != OR_SUCCESS) {
+            result = failure;
+            break;
+        }
+        if (dt_grid == DBL_MAX) break;
+        const double next = grid_time + dt_grid;
+        if (!valid_positive(dt_grid) || !isfinite(next)) {
+            result = OR_BAD_INPUTS;
+            break;
+        }
+        int integrated = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : integrated)
+        for (int c = 0; c < n; ++c) {
+            int f = OR_SUCCESS;
+            integrated += or_advance_cell(&cells[c], completed, next, dt_grid, &f);
+            if (f != OR_SUCCESS) {
+#pragma omp critical
+                if (failure == OR_SUCCESS) failure = f;
+            }
+        }
+        if (failure != OR_SUCCESS) {
+            result = failure;
+            break;
+        }
+        if (integrated == 0) break;
+        grid_time = next;
+        completed += 1;
+    }
+    *completed_out = completed;
+    return result;
+}
+
+int or_sizeof_cell(void) { return (int)sizeof(or_cell); }
