@@ -1,0 +1,213 @@
+/*
+ * warp_reactor.h -- C ABI of the MI355X (gfx950) primordial-chemistry ROS2S
+ * grid advance.
+ *
+ * The library replaces the reference's device runner: the host loop
+ * run_cells_cuda (reproducer.cpp:7791-7975) with its two kernels
+ * prepare_grid_timestep_kernel (7696-7734) and
+ * advance_collapse_gridwide_kernel (7736-7785), keeping their semantics:
+ * per-cell perturbation and free-fall dt, the grid-wide minimum dt, the
+ * gridwide-independent ROS2S advance, first-failure-wins error codes and the
+ * 1000-step cap.  All arithmetic is IEEE FP64.
+ *
+ * Conventions
+ *  - Functions return an int status: the reference's IntegratorResult values
+ *    unchanged (WR_SUCCESS = 1, negative integrator codes), or a distinct
+ *    WR_ERR_* code (<= -100) for API/HIP errors.  wr_last_error() returns the
+ *    message of the most recent error on the calling thread.
+ *  - Plain pointers and sizes only.  Host pointers unless stated.
+ *  - A context owns all device memory and one HIP stream; calls on a context
+ *    are synchronous and must come from one host thread at a time.
+ */
+#ifndef WARP_REACTOR_H
+#define WARP_REACTOR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WR_ABI_VERSION 1
+
+#define WR_NUM_SPEC 14
+#define WR_NEQS 15
+
+/* IntegratorResult (reproducer.cpp:30-37) and API errors. */
+enum wr_status {
+    WR_SUCCESS = 1,
+    WR_BAD_INPUTS = -1,
+    WR_DT_UNDERFLOW = -2,
+    WR_TOO_MANY_STEPS = -4,
+    WR_TOO_MUCH_ACCURACY_REQUESTED = -5,
+    WR_LU_DECOMPOSITION_ERROR = -7,
+    WR_ERR_HIP = -100,      /* a HIP runtime call failed */
+    WR_ERR_ARGUMENT = -101, /* invalid argument (null pointer, size) */
+    WR_ERR_CAPACITY = -102, /* more cells than the context was created for */
+    WR_ERR_IO = -103        /* file I/O error */
+};
+
+/* burn_t (reproducer.cpp:259-264): 136 bytes. */
+typedef struct wr_burn_state {
+    double rho;
+    double T;
+    double e;
+    double xn[WR_NUM_SPEC];
+} wr_burn_state;
+
+/* IntegratorStats (reproducer.cpp:7433-7441): 56 bytes. */
+typedef struct wr_integrator_stats {
+    uint64_t internal_steps;
+    uint64_t rhs_calls;
+    uint64_t jacobian_calls;
+    uint64_t decompositions;
+    uint64_t linear_solves;
+    uint64_t accepted_steps;
+    uint64_t rejected_steps;
+} wr_integrator_stats;
+
+/* CollapseState (reproducer.cpp:7455-7461): 216 bytes, same layout. */
+typedef struct wr_collapse_state {
+    wr_burn_state current;
+    double time;
+    double density_driver;
+    int32_t completed_steps;
+    int32_t reserved_;
+    wr_integrator_stats stats;
+} wr_collapse_state;
+
+/* PackedFinalState (reproducer.cpp:7488-7505): 172 bytes, little endian,
+ * no padding.  i = cell % N, j = (cell / N) % N, k = cell / N^2. */
+#pragma pack(push, 1)
+typedef struct wr_packed_final_state {
+    int32_t cell;
+    int32_t i;
+    int32_t j;
+    int32_t k;
+    int32_t completed_steps;
+    double time;
+    double density_driver;
+    double rho;
+    double T;
+    double e;
+    double xn[WR_NUM_SPEC];
+} wr_packed_final_state;
+#pragma pack(pop)
+
+/* Why a run stopped. */
+enum wr_stop_reason {
+    WR_STOP_NONE = 0,           /* still running / not run */
+    WR_STOP_STEP_LIMIT = 1,     /* max_steps outer steps completed (reproducer.cpp:7859) */
+    WR_STOP_NO_CANDIDATE = 2,   /* no cell offered a dt (7894-7896) */
+    WR_STOP_NONE_INTEGRATED = 3,/* advance integrated no cell (7954-7956) */
+    WR_STOP_FAILURE = 4         /* a cell failed; run terminal (7881-7886, 7948-7953) */
+};
+
+/* Outcome of a run, filled by wr_run_info(). */
+typedef struct wr_run_info {
+    int32_t status;             /* wr_status */
+    int32_t stop_reason;        /* wr_stop_reason */
+    int32_t completed_global_steps;
+    int32_t failed_step;        /* outer step of the failure, or -1 */
+    int32_t failed_phase;       /* 1 = prepare, 2 = advance, 0 = none */
+    int32_t reserved_;
+    double grid_time;           /* physical time of the grid after the run */
+    uint64_t cell_advances;     /* sum over steps of integrated cells */
+} wr_run_info;
+
+typedef struct wr_context wr_context;
+
+/* ---- version / errors ------------------------------------------------------ */
+int wr_abi_version(void);
+const char *wr_last_error(void);
+
+/* ---- drop-in runner ----------------------------------------------------------
+ * Replaces run_cells_cuda (reproducer.cpp:7791): uploads the cells, runs the
+ * gridwide-independent outer loop (<= 1000 steps) and downloads them.  On any
+ * HIP error prints "<action> failed: <message>" to stderr and returns
+ * WR_BAD_INPUTS, as the reference does (7688-7694). */
+int wr_run_cells(wr_collapse_state *cells, int num_cells, int perturb,
+                 int *completed_global_steps);
+
+/* ---- context ---------------------------------------------------------------- */
+int wr_context_create(int device, int capacity_cells, wr_context **out);
+void wr_context_destroy(wr_context *ctx);
+int wr_upload_cells(wr_context *ctx, const wr_collapse_state *cells, int num_cells);
+int wr_download_cells(wr_context *ctx, wr_collapse_state *cells, int num_cells);
+
+/* Run the outer loop on the resident cells from global step
+ * `completed_global_steps` and grid time `grid_time` (normally 0, 0) for at
+ * most `max_steps` steps (<= 0: 1000).  Host-side control is device resident:
+ * the host only polls a completion flag every few steps. */
+int wr_run(wr_context *ctx, int perturb, int max_steps, int completed_global_steps,
+           double grid_time);
+int wr_run_info_get(const wr_context *ctx, wr_run_info *out);
+
+/* One kernel of the outer step each, for callers that drive the loop:
+ * prepare (reproducer.cpp:7696) + device min reduction (7888-7896);
+ * *dt_grid is DBL_MAX when no cell offers a candidate.  `*limiting_cell` is
+ * the lowest cell index holding the minimum (or -1). */
+int wr_prepare(wr_context *ctx, int completed_global_steps, double grid_time, int step,
+               int perturb, double *dt_grid, int *limiting_cell);
+/* advance (reproducer.cpp:7736); *integrated = number of cells integrated. */
+int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_time,
+               double dt_grid, int *integrated);
+
+/* Time the advance kernel launches of subsequent wr_run calls with HIP events
+ * recorded on the context's stream (off by default). */
+int wr_set_timing(wr_context *ctx, int enable);
+/* Total device time (ms) of the advance launches of the last timed wr_run
+ * that integrated cells, and their count. */
+int wr_last_advance_timing(const wr_context *ctx, double *total_ms, int *launches);
+
+/* ---- device building blocks (batched; for verification) --------------------
+ * y: n x 15 row-major host arrays; outputs likewise.  Each runs the same
+ * device functions the advance kernel uses. */
+int wr_eval_rhs(wr_context *ctx, const double *y, int n, double *dydt);
+int wr_eval_jac(wr_context *ctx, const double *y, int n, double *jac /* n x 225 */);
+int wr_eval_lu(wr_context *ctx, double *a /* n x 225, in/out */, int n, int32_t *ipvt /* n x 15 */,
+               int32_t *info /* n */);
+int wr_eval_lu_solve(wr_context *ctx, const double *lu, const int32_t *ipvt, int n,
+                     double *x /* n x 15, in/out */);
+/* RODAS<PrimordialChem>::integrate from t = 0 to tout[i] with first step
+ * dt0[i] and the collapse tolerances; counters n x 7 = n_step, n_rhs, n_jac,
+ * n_decomp, n_solve, n_accept, n_reject; t_end/dt_end as the integrator
+ * leaves them; result n codes.  max_steps <= 0: 1e7. */
+int wr_eval_integrate(wr_context *ctx, double *y, const double *tout, const double *dt0, int n,
+                      int max_steps, int32_t *counters, double *t_end, double *dt_end,
+                      int32_t *result);
+
+/* ---- initial states, packed final states, comparison (host) ------------------ */
+/* make_collapse_state (reproducer.cpp:7533-7546) for n cells. */
+int wr_make_collapse_states(wr_collapse_state *cells, int num_cells);
+/* make_packed_final_state (8289-8305). */
+int wr_pack_final_states(const wr_collapse_state *cells, int num_cells, int grid_dim,
+                         wr_packed_final_state *out);
+/* write_final_states (8357-8378), rotating an existing file to
+ * PATH.old.NNNNNN first (8307-8355). */
+int wr_write_final_states(const wr_collapse_state *cells, int num_cells, int grid_dim,
+                          const char *path);
+/* read_final_states (8380-8414): exact size check. */
+int wr_read_final_states(const char *path, int expected_records, wr_packed_final_state *out);
+
+typedef struct wr_comparison_summary {
+    int32_t pass;
+    int32_t failures;           /* number of failing fields */
+    double max_deuterium_species_rel_error;
+    double max_non_deuterium_species_rel_error;
+    double max_thermodynamic_rel_error;
+    double max_rho_rel_error;
+    char first_failure[512];
+} wr_comparison_summary;
+
+/* compare_final_states (8416-8549) against a packed file; when csv_path is
+ * non-null and the comparison fails, writes the failures CSV (8100-8142). */
+int wr_compare_final_states(const wr_collapse_state *cells, int num_cells, int grid_dim,
+                            const char *path, const char *csv_path,
+                            wr_comparison_summary *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WARP_REACTOR_H */

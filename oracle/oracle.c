@@ -595,3 +595,6 @@ int or_run_gridwide(or_cell *cells, int n, int perturb, int max_steps, int *comp
 }
 
 int or_sizeof_cell(void) { return (int)sizeof(or_cell); }
+
+/* eos_rt on a burn_t laid out as rho, T, e, xn[14] (reproducer.cpp:306-309) */
+void or_eos_rt(double *burn) { eos_rt((or_burn_t *)burn); }

@@ -227,6 +227,13 @@ double ref_collapse_timestep(const void* cell) {
     return collapse_timestep(*static_cast<const CollapseState*>(cell));
 }
 
+void ref_eos_rt(double* burn) {
+    ref_pc::burn_t b;
+    std::memcpy(&b, burn, sizeof(b));
+    ref_pc::eos_rt(b);
+    std::memcpy(burn, &b, sizeof(b));
+}
+
 void ref_post_step(double* burn) {
     ref_pc::burn_t b;
     std::memcpy(&b, burn, sizeof(b));

@@ -1,0 +1,148 @@
+"""Test-side loaders of the CPU oracle (oracle/_build/liboracle.so) and of the
+reference's own CPU code (oracle/_ref/libwr_ref.so).  Both are checkers; the
+product never loads them."""
+import ctypes
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+REF_SO = os.path.join(REPO, "oracle", "_ref", "libwr_ref.so")
+REF_CLI = os.path.join(REPO, "oracle", "_ref", "reproducer_cpu")
+
+_VP, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+def _p(a):
+    return a.ctypes.data_as(_VP)
+
+
+class _Cpu:
+    prefix = ""
+
+    def __init__(self, path):
+        self.lib = ctypes.CDLL(path)
+        p = self.prefix
+        sig = {
+            "rhs": (None, [_VP, _VP]), "jac": (None, [_VP, _VP]),
+            "temperature_from_y": (_D, [_VP]),
+            "lu": (_I, [_VP, _VP]), "lu_solve": (None, [_VP, _VP, _VP]),
+            "integrate": (_I, [_VP, _D, _D, _I, _VP, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+            "burn": (_I, [_VP, _D, _VP]),
+            "make_collapse_state": (None, [_VP]),
+            "splitmix64": (ctypes.c_uint64, [ctypes.c_uint64]),
+            "perturbation_factor": (_D, [_I, _I]),
+            "apply_perturbation": (None, [_VP, _I, _I, _I]),
+            "collapse_timestep": (_D, [_VP]),
+            "post_step": (None, [_VP]),
+            "eos_rt": (None, [_VP]),
+            "prepare_cell": (_D, [_VP, _I, _I, _D, _I, _I, ctypes.POINTER(_I)]),
+            "advance_cell": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
+            "run_gridwide": (_I, [_VP, _I, _I, _I, ctypes.POINTER(_I)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(self.lib, p + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, "_" + name, fn)
+
+    def rhs(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(15)
+        self._rhs(_p(y), _p(out))
+        return out
+
+    def jac(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros((15, 15))
+        self._jac(_p(y), _p(out))
+        return out
+
+    def temperature_from_y(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        return self._temperature_from_y(_p(y))
+
+    def lu(self, a):
+        a = np.array(a, dtype=np.float64).reshape(15, 15).copy()
+        ip = np.zeros(15, dtype=np.int32)
+        info = self._lu(_p(a), _p(ip))
+        return a, ip, info
+
+    def lu_solve(self, lu, ip, x):
+        lu = np.ascontiguousarray(lu, dtype=np.float64)
+        ip = np.ascontiguousarray(ip, dtype=np.int32)
+        x = np.array(x, dtype=np.float64).copy()
+        self._lu_solve(_p(lu), _p(ip), _p(x))
+        return x
+
+    def integrate(self, y, tout, dt0, max_steps=0):
+        y = np.array(y, dtype=np.float64).copy()
+        cnt = np.zeros(7, dtype=np.int32)
+        te, de = _D(), _D()
+        r = self._integrate(_p(y), tout, dt0, max_steps, _p(cnt), ctypes.byref(te), ctypes.byref(de))
+        return {"y": y, "counters": cnt, "t_end": te.value, "dt_end": de.value, "result": r}
+
+    def burn(self, burn, dt, stats):
+        burn = np.array(burn, dtype=np.float64).copy()
+        stats = np.array(stats, dtype=np.uint64).copy()
+        r = self._burn(_p(burn), dt, _p(stats))
+        return r, burn, stats
+
+    def make_cells(self, n, dtype):
+        cells = np.zeros(n, dtype=dtype)
+        for i in range(n):
+            self._make_collapse_state(ctypes.c_void_p(cells.ctypes.data + i * dtype.itemsize))
+        cells["reserved_"] = 0
+        return cells
+
+    def burn_to_y(self, burn):
+        """y = (xn, e) with e from eos_rt, as burn_ros2s builds it (7595-7606)."""
+        b = np.array(burn, dtype=np.float64).copy()
+        self._eos_rt(_p(b))
+        return np.concatenate([b[3:], [b[2]]])
+
+    def splitmix64(self, v):
+        return self._splitmix64(v)
+
+    def perturbation_factor(self, cell, step):
+        return self._perturbation_factor(cell, step)
+
+    def collapse_timestep(self, cell):
+        return self._collapse_timestep(_p(cell))
+
+    def prepare_cell(self, cell, index, completed, grid_time, step, perturb):
+        f = _I(1)
+        dt = self._prepare_cell(_p(cell), index, completed, grid_time, step, int(perturb),
+                                ctypes.byref(f))
+        return dt, f.value
+
+    def advance_cell(self, cell, completed, next_time, dt_grid):
+        f = _I(1)
+        r = self._advance_cell(_p(cell), completed, next_time, dt_grid, ctypes.byref(f))
+        return r, f.value
+
+    def run_gridwide(self, cells, perturb=True, max_steps=0):
+        done = _I(0)
+        r = self._run_gridwide(_p(cells), len(cells), int(perturb), max_steps, ctypes.byref(done))
+        cells["reserved_"] = 0
+        return r, done.value
+
+
+class Oracle(_Cpu):
+    prefix = "or_"
+
+    def __init__(self):
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (make -C oracle)")
+        super().__init__(ORACLE_SO)
+
+
+class Reference(_Cpu):
+    prefix = "ref_"
+
+    @classmethod
+    def load(cls):
+        if not os.path.exists(REF_SO):
+            return None
+        return cls(REF_SO)

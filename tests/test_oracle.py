@@ -1,0 +1,265 @@
+"""Pinning of the CPU oracle (oracle/oracle.c).
+
+1. Golden vectors written by the reference's own compiled code
+   (tests/golden/make_golden.py): the oracle must reproduce them bit for bit.
+2. Live comparison with the reference CPU build (oracle/_ref) where it is
+   built, on fresh seeded inputs.
+3. What the mathematics fixes, independent of the reference: the
+   SplitMix64 published output stream, mass and charge conservation of the
+   network RHS, the energy column of J against a finite difference of f in e,
+   the LU factorisation reproducing P A = L U with the pivots a brute-force
+   search picks, the free-fall formula, and the grid-1 known answer.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from states import lu_test_matrices, random_states
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def bits(a):
+    """Bit patterns, with every NaN mapped to one pattern: IEEE 754 leaves the
+    sign and payload of a propagated NaN to the implementation."""
+    a = np.array(a, dtype=np.float64)
+    a[np.isnan(a)] = np.nan
+    return a.view(np.uint64)
+
+
+# ---- golden vectors --------------------------------------------------------------
+
+def test_rhs_jac_golden(oracle):
+    g = load("rhs_jac.npz")
+    for y, f, J in zip(g["y"], g["f"], g["J"]):
+        assert np.array_equal(bits(oracle.rhs(y)), bits(f))
+        assert np.array_equal(bits(oracle.jac(y)), bits(J))
+
+
+def test_lu_golden(oracle):
+    g = load("lu.npz")
+    for a, lu, ip, info, b, x in zip(g["a"], g["lu"], g["ipvt"], g["info"], g["b"], g["x"]):
+        got, p, i = oracle.lu(a)
+        assert i == info
+        assert np.array_equal(p, ip)
+        assert np.array_equal(bits(got), bits(lu))
+        if info == 0:
+            assert np.array_equal(bits(oracle.lu_solve(got, p, b)), bits(x))
+
+
+def test_integrate_golden(oracle):
+    g = load("integrate.npz")
+    assert len(g["y0"]) >= 16
+    for k in range(len(g["y0"])):
+        o = oracle.integrate(g["y0"][k], g["tout"][k], g["dt0"][k])
+        assert o["result"] == g["result"][k]
+        assert np.array_equal(o["counters"], g["counters"][k])
+        assert np.array_equal(bits(o["y"]), bits(g["y"][k]))
+        assert o["t_end"] == g["t_end"][k] and o["dt_end"] == g["dt_end"][k]
+    o = oracle.integrate(g["y0"][0], g["tout"][0], g["dt0"][0], max_steps=3)
+    assert o["result"] == g["forced_result"] == -4  # TOO_MANY_STEPS
+    assert np.array_equal(o["counters"], g["forced_counters"])
+    assert np.array_equal(bits(o["y"]), bits(g["forced_y"]))
+
+
+def test_driver_golden(oracle, wr):
+    g = load("driver.npz")
+    for s, want in zip(g["splitmix_in"], g["splitmix_out"]):
+        assert oracle.splitmix64(int(s)) == int(want)
+    for i, c in enumerate(g["pf_cells"]):
+        for j, s in enumerate(g["pf_steps"]):
+            assert oracle.perturbation_factor(int(c), int(s)) == g["pf"][i, j]
+    mask = np.ones(216, bool)
+    mask[156:160] = False  # padding after completed_steps
+    for key, n, perturb in (("grid1", 1, True), ("grid2", 8, True), ("grid3_noperturb", 27, False)):
+        cells = oracle.make_cells(n, wr.CELL_DTYPE)
+        r, done = oracle.run_gridwide(cells, perturb)
+        assert r == 1
+        assert done == int(g[key.replace("_noperturb", "") + "_steps"])
+        want = g[key].reshape(n, 216)[:, mask]
+        assert np.array_equal(cells.view(np.uint8).reshape(n, 216)[:, mask], want), key
+
+
+def test_prepare_semantics_golden(oracle, wr):
+    """Terminal small-dt update applied once (1e-10 -> 1.1e-10), all-zero
+    densities -> BAD_INPUTS, density limit stops before the burn
+    (tests/integration/test_grid_timestep_reference.mojo:33-71)."""
+    g = load("driver.npz")
+    term = oracle.make_cells(1, wr.CELL_DTYPE)
+    term["current"]["xn"][0] = 0.0
+    term["current"]["xn"][0][2] = 1.0e30
+    term["density_driver"] = 1.0e-10
+    dt, f = oracle.prepare_cell(term, 0, 0, 0.0, 0, False)
+    assert dt == np.finfo(np.float64).max and f == 1
+    assert abs(term["density_driver"][0] - 1.1e-10) < 1e-25
+    assert term.view(np.uint8).tobytes() == g["terminal_cell"].tobytes()
+    bad = oracle.make_cells(1, wr.CELL_DTYPE)
+    bad["current"]["xn"][0] = 0.0
+    dt, f = oracle.prepare_cell(bad, 0, 0, 0.0, 0, False)
+    assert f == -1 == int(g["bad_failure"])
+    lim = oracle.make_cells(1, wr.CELL_DTYPE)
+    lim["density_driver"] = 1.99e-6
+    dtl = oracle.collapse_timestep(lim)
+    r, f = oracle.advance_cell(lim, 0, dtl, dtl)
+    assert r == 0 == int(g["limit_integrated"]) and f == 1
+    assert lim["density_driver"][0] > 2.0e-6 and lim["completed_steps"][0] == 0
+    assert lim.view(np.uint8).tobytes() == g["limit_cell"].tobytes()
+
+
+# ---- live comparison with the reference build ----------------------------------------
+
+def test_rhs_jac_match_reference_live(oracle, ref):
+    for y in random_states(300, oracle, seed=7, t_range=(0.0, 5.5)):
+        assert np.array_equal(bits(oracle.rhs(y)), bits(ref.rhs(y)))
+        assert np.array_equal(bits(oracle.jac(y)), bits(ref.jac(y)))
+
+
+def test_gridwide_match_reference_live(oracle, ref, wr):
+    for n, perturb in ((5, True), (12, False)):
+        a = oracle.make_cells(n, wr.CELL_DTYPE)
+        b = ref.make_cells(n, wr.CELL_DTYPE)
+        ra, da = oracle.run_gridwide(a, perturb)
+        rb, db = ref.run_gridwide(b, perturb)
+        assert (ra, da) == (rb, db)
+        assert a.tobytes() == b.tobytes()
+
+
+# ---- closed forms and invariants -------------------------------------------------------
+
+def test_splitmix64_published_stream(oracle):
+    """SplitMix64 (Steele, Lea, Flood 2014) seeded with 1234567: the reference's
+    splitmix64(v) is one step of the generator from state v, so the stream is
+    splitmix64(seed + k * gamma)."""
+    gamma = 0x9E3779B97F4A7C15
+    want = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+            4593380528125082431, 16408922859458223821]
+    got = [oracle.splitmix64((1234567 + k * gamma) % 2**64) for k in range(5)]
+    assert got == want
+
+
+# charges of E, H+, H, H-, D+, D, H2+, D-, H2, HD+, HD, He++, He+, He
+CHARGE = np.array([-1, 1, 0, -1, 1, 0, 1, -1, 0, 1, 0, 2, 1, 0], dtype=float)
+MASS = np.array([9.10938188e-28, 1.67262158e-24, 1.67353251819e-24, 1.67444345638e-24,
+                 3.34512158e-24, 3.34603251819e-24, 3.34615409819e-24, 3.34694345638e-24,
+                 3.34706503638e-24, 5.01865409819e-24, 5.01956503638e-24, 6.69024316e-24,
+                 6.69115409819e-24, 6.69206503638e-24])
+
+
+def test_rhs_conserves_mass_and_charge(oracle):
+    checked = 0
+    for y in random_states(200, oracle, seed=8, t_range=(1.5, 4.5)):
+        f = oracle.rhs(y)[:14]
+        if not np.all(np.isfinite(f)):
+            continue  # floored-zero states can overflow a rate; not a balance test
+        checked += 1
+        dm = MASS * f
+        dq = CHARGE * f
+        assert abs(dm.sum()) <= 1e-12 * max(np.abs(dm).sum(), 1e-300)
+        assert abs(dq.sum()) <= 1e-12 * max(np.abs(dq).sum(), 1e-300)
+    assert checked >= 150
+
+
+def test_jacobian_energy_column_is_df_de(oracle, wr):
+    """Column 15 of J is d f / d e at fixed composition (chain rule through T),
+    checked on the burn inputs of a collapse run (T ~ 300 .. 2300 K).  The
+    reference J is analytic but inexact at the 3e-7 level; the species
+    columns are not exact derivatives at all and are not checked here."""
+    from states import trajectory_cases
+    worst = 0.0
+    cases = trajectory_cases(oracle, wr.CELL_DTYPE, n_cells=2, steps=(20, 100, 200, 300, 400))
+    assert len(cases) >= 8
+    for burn, _ in cases:
+        y = oracle.burn_to_y(burn)
+        J = oracle.jac(y)
+        h = 1e-5 * y[14]
+        yp, ym = y.copy(), y.copy()
+        yp[14] += h
+        ym[14] -= h
+        fd = (oracle.rhs(yp) - oracle.rhs(ym)) / (2 * h)
+        scale = np.abs(fd) + 1e-30 * np.abs(fd).max()
+        big = np.abs(fd) > 1e-6 * np.abs(fd).max()
+        worst = max(worst, (np.abs(J[:, 14] - fd) / scale)[big].max())
+    assert worst < 1e-5
+
+
+def _brute_force_lu(a):
+    """Textbook Gaussian elimination with the reference's pivot rule, storing
+    the permutation explicitly; returns P, L, U with P A = L U."""
+    n = a.shape[0]
+    u = a.astype(float).copy()
+    perm = np.arange(n)
+    l = np.eye(n)
+    for k in range(n - 1):
+        col = np.abs(u[k:, k])
+        p = k + int(np.argmax(col))  # argmax returns the first maximum
+        if u[p, k] == 0.0:
+            continue
+        u[[k, p], :] = u[[p, k], :]
+        l[[k, p], :k] = l[[p, k], :k]
+        perm[[k, p]] = perm[[p, k]]
+        for i in range(k + 1, n):
+            l[i, k] = u[i, k] / u[k, k]
+            u[i, :] -= l[i, k] * u[k, :]
+    return perm, l, u
+
+
+def test_lu_reconstructs_and_pivots_like_brute_force(oracle):
+    rng = np.random.default_rng(10)
+    mats = list(lu_test_matrices()[0][[0, 3, 4, 5, 6, 7]]) + list(rng.standard_normal((20, 15, 15)))
+    for a in mats:
+        lu, ip, info = oracle.lu(a)
+        assert info == 0
+        perm, l_bf, u_bf = _brute_force_lu(a)
+        # replay the LINPACK interchanges on the identity to get P
+        p = np.arange(15)
+        for k in range(14):
+            p[[k, ip[k]]] = p[[ip[k], k]]
+        assert np.array_equal(p, perm)
+        U = np.triu(lu)
+        assert np.allclose(U, np.triu(u_bf), rtol=1e-10, atol=1e-12 * np.abs(a).max())
+        # solve check: residual of A x = b
+        b = rng.standard_normal(15)
+        x = oracle.lu_solve(lu, ip, b)
+        assert np.linalg.norm(a @ x - b) <= 1e-11 * (np.linalg.norm(a) * np.linalg.norm(x) + np.linalg.norm(b))
+
+
+def test_lu_singular_cases(oracle):
+    mats, _ = lu_test_matrices()
+    assert oracle.lu(mats[1])[2] == 15  # zero last row
+    z = np.eye(15)
+    z[:, 4] = 0.0
+    assert oracle.lu(z)[2] == 5  # zero column 5: info = k + 1
+
+
+def test_free_fall_timestep_closed_form(oracle, wr):
+    cells = oracle.make_cells(1, wr.CELL_DTYPE)
+    rho = float(np.dot(cells["current"]["xn"][0], MASS))
+    tff = np.sqrt(3.0 * np.pi / (32.0 * 6.674e-8 * rho))
+    assert abs(oracle.collapse_timestep(cells) - 0.1 * tff) <= 1e-14 * 0.1 * tff
+
+
+def test_grid1_known_answer(oracle, wr):
+    """438 global steps and the reference binary's counters
+    (8186 steps, 24558 RHS, 5994 J, 5994/103 accepted/rejected)."""
+    cells = oracle.make_cells(1, wr.CELL_DTYPE)
+    r, done = oracle.run_gridwide(cells, True)
+    assert r == 1 and done == 438
+    st = cells["stats"][0]
+    assert (st["internal_steps"], st["rhs_calls"], st["jacobian_calls"]) == (8186, 24558, 5994)
+    assert (st["accepted_steps"], st["rejected_steps"]) == (5994, 103)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "reproducer_cpu")),
+                    reason="reference CLI not built")
+def test_reference_cli_grid1_prints_438(tmp_path):
+    import subprocess
+    import oracle_lib
+    out = subprocess.run([oracle_lib.REF_CLI, "--grid", "1", "--no-compare-final-state"],
+                         cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0
+    assert "completed global collapse steps: 438" in out.stdout

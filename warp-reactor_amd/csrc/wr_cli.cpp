@@ -1,0 +1,259 @@
+// wr_reproducer: command-line front end with the reference reproducer's
+// interface and output (reproducer.cpp:8201-8277, 8591-8703) for the
+// gridwide-independent policy on the gfx950 library.
+//
+//   wr_reproducer [--grid N] [--perturb|--no-perturb]
+//                 [--compare-final-state FILE|--no-compare-final-state]
+//
+// Writes final_states_grid<N>_hip.bin (172-byte records) for N > 1, prints
+// the final state for N = 1, exit status 0 on success and 1 on failure.
+#include <errno.h>
+#include <limits.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "warp_reactor.h"
+
+namespace {
+
+const char* const kSpeciesNames[WR_NUM_SPEC] = {"E",  "Hp", "H",  "Hm",  "Dp",  "D",   "H2p",
+                                                "Dm", "H2", "HDp", "HD", "HEpp", "HEp", "HE"};
+const char* const kBackend = "hip";
+
+struct Options {
+    int grid_dim = 64;
+    bool perturb = true;
+    bool show_help = false;
+    std::string compare_path;
+};
+
+void print_usage(const char* program) {
+    fprintf(stderr,
+            "usage: %s [--grid N] [--perturb|--no-perturb]"
+            " [--compare-final-state FILE|--no-compare-final-state]\n",
+            program);
+}
+
+bool parse_positive_int(const char* text, int& value) {
+    errno = 0;
+    char* end = nullptr;
+    const long parsed = strtol(text, &end, 10);
+    if (errno != 0 || end == text || *end != '\0' || parsed <= 0 || parsed > INT_MAX) return false;
+    value = static_cast<int>(parsed);
+    return true;
+}
+
+bool parse_args(int argc, char** argv, Options& o) {
+    const std::string prefix = "--compare-final-state=";
+    for (int i = 1; i < argc; ++i) {
+        const std::string arg = argv[i];
+        if (arg == "--help" || arg == "-h") {
+            o.show_help = true;
+            return true;
+        }
+        if (arg == "--grid") {
+            if (i + 1 >= argc || !parse_positive_int(argv[++i], o.grid_dim)) {
+                print_usage(argv[0]);
+                return false;
+            }
+            continue;
+        }
+        if (arg == "--perturb") {
+            o.perturb = true;
+            continue;
+        }
+        if (arg == "--no-perturb") {
+            o.perturb = false;
+            continue;
+        }
+        if (arg == "--compare-final-state") {
+            if (i + 1 >= argc) {
+                print_usage(argv[0]);
+                return false;
+            }
+            o.compare_path = argv[++i];
+            continue;
+        }
+        if (arg == "--no-compare-final-state") {
+            o.compare_path.clear();
+            continue;
+        }
+        if (arg.rfind(prefix, 0) == 0) {
+            o.compare_path = arg.substr(prefix.size());
+            if (o.compare_path.empty()) {
+                print_usage(argv[0]);
+                return false;
+            }
+            continue;
+        }
+        print_usage(argv[0]);
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
+long double median_of(std::vector<T> v, T& mn, T& mx) {
+    std::sort(v.begin(), v.end());
+    mn = v.front();
+    mx = v.back();
+    const size_t n = v.size();
+    return (n % 2 == 1) ? static_cast<long double>(v[n / 2])
+                        : (static_cast<long double>(v[n / 2 - 1]) + static_cast<long double>(v[n / 2])) / 2.0L;
+}
+
+void print_value_summary(const char* label, std::vector<long double> v) {
+    long double mn, mx;
+    const long double med = median_of(std::move(v), mn, mx);
+    printf("%s: [%.17Lg, %.17Lg, %.17Lg]\n", label, mn, med, mx);
+}
+
+void print_scientific_summary(const char* label, std::vector<long double> v) {
+    long double mn, mx;
+    const long double med = median_of(std::move(v), mn, mx);
+    printf("%s: [%.17Le, %.17Le, %.17Le]\n", label, mn, med, mx);
+}
+
+struct CounterSummary {
+    uint64_t min, max;
+    long double median;
+};
+
+CounterSummary summarize_counter(const std::vector<wr_collapse_state>& cells, size_t which) {
+    std::vector<uint64_t> v;
+    v.reserve(cells.size());
+    for (const auto& c : cells) v.push_back(reinterpret_cast<const uint64_t*>(&c.stats)[which]);
+    CounterSummary s;
+    s.median = median_of(std::move(v), s.min, s.max);
+    return s;
+}
+
+int finish(bool pass, bool emit_status) {
+    fflush(stderr);
+    if (emit_status) printf("\n%s\n", pass ? "PASS" : "FAIL");
+    fflush(stdout);
+    return pass ? 0 : 1;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Options o;
+    if (!parse_args(argc, argv, o)) return finish(false, false);
+    if (o.show_help) {
+        print_usage(argv[0]);
+        return finish(true, false);
+    }
+    const long long n3 = static_cast<long long>(o.grid_dim) * o.grid_dim * o.grid_dim;
+    if (n3 <= 0 || n3 > INT_MAX) {
+        fprintf(stderr, "grid dimension is too large: %d\n", o.grid_dim);
+        return finish(false, false);
+    }
+    const int num_cells = static_cast<int>(n3);
+    std::vector<wr_collapse_state> cells(static_cast<size_t>(num_cells));
+    wr_make_collapse_states(cells.data(), num_cells);
+
+    int completed = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int result = wr_run_cells(cells.data(), num_cells, o.perturb ? 1 : 0, &completed);
+    const auto t1 = std::chrono::steady_clock::now();
+    const double elapsed = std::chrono::duration<double>(t1 - t0).count();
+    if (result != WR_SUCCESS) return finish(false, false);
+
+    wr_integrator_stats total{};
+    for (const auto& c : cells) {
+        total.internal_steps += c.stats.internal_steps;
+        total.rhs_calls += c.stats.rhs_calls;
+        total.jacobian_calls += c.stats.jacobian_calls;
+        total.decompositions += c.stats.decompositions;
+        total.linear_solves += c.stats.linear_solves;
+        total.accepted_steps += c.stats.accepted_steps;
+        total.rejected_steps += c.stats.rejected_steps;
+    }
+    const wr_collapse_state& rep = cells.front();
+
+    printf("Primordial chemistry collapse grid with ROS2S\n");
+    printf("grid: %d^3 (%d cells)\n", o.grid_dim, num_cells);
+    printf("perturbations: %s\n", o.perturb ? "enabled" : "disabled");
+    printf("driver policy: gridwide-independent\n");
+    printf("completed global collapse steps: %d\n", completed);
+    if (num_cells == 1) {
+        printf("representative cell completed steps: %d\n", rep.completed_steps);
+        printf("representative cell physical time: %.17Le\n", static_cast<long double>(rep.time));
+        printf("representative cell density driver: %.17g\n", rep.density_driver);
+    } else {
+        std::vector<long double> steps, times, drivers;
+        for (const auto& c : cells) {
+            steps.push_back(c.completed_steps);
+            times.push_back(c.time);
+            drivers.push_back(c.density_driver);
+        }
+        print_value_summary("cell completed steps", std::move(steps));
+        print_scientific_summary("cell physical time", std::move(times));
+        print_value_summary("cell density driver", std::move(drivers));
+    }
+    printf("wall time: %.17g s\n", elapsed);
+    if (num_cells == 1) {
+        printf("ROS2S internal steps: %llu\n", (unsigned long long)total.internal_steps);
+        printf("ROS2S rhs calls: %llu\n", (unsigned long long)total.rhs_calls);
+        printf("ROS2S jacobian calls: %llu\n", (unsigned long long)total.jacobian_calls);
+        printf("ROS2S decompositions: %llu\n", (unsigned long long)total.decompositions);
+        printf("ROS2S linear solves: %llu\n", (unsigned long long)total.linear_solves);
+        printf("ROS2S accepted/rejected: %llu/%llu\n\n", (unsigned long long)total.accepted_steps,
+               (unsigned long long)total.rejected_steps);
+    } else {
+        const char* labels[5] = {"ROS2S internal steps", "ROS2S rhs calls", "ROS2S jacobian calls",
+                                 "ROS2S decompositions", "ROS2S linear solves"};
+        for (size_t k = 0; k < 5; ++k) {
+            const CounterSummary s = summarize_counter(cells, k);
+            printf("%s: [%llu, %.17Lg, %llu]\n", labels[k], (unsigned long long)s.min, s.median,
+                   (unsigned long long)s.max);
+        }
+        const CounterSummary a = summarize_counter(cells, 5), r = summarize_counter(cells, 6);
+        printf("ROS2S accepted/rejected: [%llu/%llu, %.17Lg/%.17Lg, %llu/%llu]\n\n",
+               (unsigned long long)a.min, (unsigned long long)r.min, a.median, r.median,
+               (unsigned long long)a.max, (unsigned long long)r.max);
+    }
+
+    bool comparison_pass = true, comparison_done = false;
+    if (!o.compare_path.empty()) {
+        comparison_done = true;
+        wr_comparison_summary sum;
+        const std::string csv = std::string("final_state_comparison_failures_") + kBackend + ".csv";
+        const int st = wr_compare_final_states(cells.data(), num_cells, o.grid_dim,
+                                               o.compare_path.c_str(), csv.c_str(), &sum);
+        if (st != WR_SUCCESS) {
+            comparison_pass = false;
+        } else {
+            printf("final-state comparison file: %s\n", o.compare_path.c_str());
+            printf("final-state comparison: %s (max deuterium-bearing species rel error %.17Le, max "
+                   "non-deuterium-bearing species rel error %.17Le, max thermodynamic rel error "
+                   "%.17Le, max rho rel error %.17Le)\n",
+                   sum.pass ? "PASS" : "FAIL", (long double)sum.max_deuterium_species_rel_error,
+                   (long double)sum.max_non_deuterium_species_rel_error,
+                   (long double)sum.max_thermodynamic_rel_error, (long double)sum.max_rho_rel_error);
+            if (!sum.pass && sum.first_failure[0]) printf("first comparison failure: %s\n", sum.first_failure);
+            comparison_pass = sum.pass != 0;
+        }
+    }
+
+    if (o.grid_dim > 1) {
+        const std::string file = "final_states_grid" + std::to_string(o.grid_dim) + "_" + kBackend + ".bin";
+        if (wr_write_final_states(cells.data(), num_cells, o.grid_dim, file.c_str()) != WR_SUCCESS)
+            return finish(false, comparison_done);
+        printf("final states: %s (%d packed records, %zu bytes each)\n", file.c_str(), num_cells,
+               sizeof(wr_packed_final_state));
+    } else {
+        printf("rho: %.17g\n", rep.current.rho);
+        printf("T: %.17g\n", rep.current.T);
+        printf("Eint: %.17g\n", rep.current.e);
+        for (int n = 0; n < WR_NUM_SPEC; ++n) printf("%s: %.17g\n", kSpeciesNames[n], rep.current.xn[n]);
+    }
+    return finish(comparison_pass, comparison_done);
+}

@@ -1,0 +1,314 @@
+// Device-side building blocks of the primordial-chemistry ROS2S grid advance
+// for gfx950: EOS and species helpers, the generated network (RHS + Jacobian),
+// dense 15x15 LU with partial pivoting, the ROS2S integrator and the per-cell
+// prepare/advance bodies.  All arithmetic is FP64 and every operation follows
+// the reference's operand order (compiled with -ffp-contract=off), so results
+// differ from the reference CPU build only through the device libm
+// (exp/log/cbrt).
+//
+// Layout: per-cell state is structure-of-arrays (one array per field, cell
+// index fastest), so a wavefront's 64 lanes touch 512 contiguous bytes per
+// field.  The per-cell Jacobian and LU workspaces are SoA too: element (i, j)
+// of cell c lives at base[(i * 15 + j) * stride + c].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wr_eos.hpp"
+
+namespace wr {
+
+// ---- generated network --------------------------------------------------------
+#define CHEM_FN __device__ __forceinline__
+#define CHEM_SINK_PARAMS , double* __restrict__ jsink, int jstride
+#define CHEM_JAC_STORE(r, c, v) (jsink[((r) * kNeqs + (c)) * jstride] = (v))
+#include "_gen/chem_network.h"
+#undef CHEM_JAC_STORE
+#undef CHEM_SINK_PARAMS
+#undef CHEM_FN
+
+// f(y) (PrimordialChem::rhs, reproducer.cpp:7040-7050)
+__device__ __forceinline__ void rhs(const double (&y)[kNeqs], double (&f)[kNeqs]) {
+    double X[kNumSpec];
+    const double T = state_from_y(y, X);
+    chem_rhs(T, X, f);
+}
+
+// J(y) into the SoA workspace and f(y) (reproducer.cpp:7052-7060 + 7040-7050)
+__device__ __forceinline__ void jac_rhs(const double (&y)[kNeqs], double (&f)[kNeqs],
+                                        double* __restrict__ J, int stride) {
+    double X[kNumSpec];
+    const double T = state_from_y(y, X);
+    chem_fused(T, X, f, J, stride);
+}
+
+// ---- LU, LINPACK conventions (reproducer.cpp:96-151) ----------------------------
+// A: this cell's matrix in the SoA workspace, element (i, j) at A[(i*15+j)*stride].
+// Column k is held in registers for the pivot search, exchange and scaling;
+// the trailing update streams rows through memory.  The pivot is the first
+// strict maximum of |a_ik|; rows are exchanged from column k on (multipliers
+// of earlier columns stay in place) and multipliers are stored negated.
+__device__ __forceinline__ int lu_factor(double* __restrict__ A, int stride, int (&ipvt)[kNeqs]) {
+    int info = 0;
+#pragma unroll
+    for (int k = 0; k < kNeqs - 1; ++k) {
+        double col[kNeqs];
+#pragma unroll
+        for (int i = k; i < kNeqs; ++i) col[i] = A[(i * kNeqs + k) * stride];
+        int p = k;
+        double best = fabs(col[k]);
+#pragma unroll
+        for (int i = k + 1; i < kNeqs; ++i) {
+            const double v = fabs(col[i]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        ipvt[k] = p;
+        double piv = col[k];
+#pragma unroll
+        for (int i = k + 1; i < kNeqs; ++i) piv = (p == i) ? col[i] : piv;
+        if (piv == 0.0) {
+            info = k + 1;
+            continue;
+        }
+        if (p != k) {
+            // exchange rows k and p from column k+1 on; column k in registers
+#pragma unroll
+            for (int j = k + 1; j < kNeqs; ++j) {
+                double* rp = A + (p * kNeqs + j) * stride;
+                double* rk = A + (k * kNeqs + j) * stride;
+                const double tp = *rp;
+                *rp = *rk;
+                *rk = tp;
+            }
+            const double ck = col[k];
+#pragma unroll
+            for (int i = k + 1; i < kNeqs; ++i) col[i] = (p == i) ? ck : col[i];
+            col[k] = piv;
+            A[(k * kNeqs + k) * stride] = piv;
+        }
+        const double m = -1.0 / col[k];
+#pragma unroll
+        for (int i = k + 1; i < kNeqs; ++i) {
+            col[i] *= m;
+            A[(i * kNeqs + k) * stride] = col[i];
+        }
+#pragma unroll
+        for (int j = k + 1; j < kNeqs; ++j) {
+            const double t = A[(k * kNeqs + j) * stride];
+#pragma unroll
+            for (int i = k + 1; i < kNeqs; ++i) {
+                double* a = A + (i * kNeqs + j) * stride;
+                *a += t * col[i];
+            }
+        }
+    }
+    ipvt[kNeqs - 1] = kNeqs - 1;
+    if (A[(kNeqs * kNeqs - 1) * stride] == 0.0) info = kNeqs;
+    return info;
+}
+
+// lu_solve (reproducer.cpp:154-181).  Row exchanges of x are done with selects
+// so x stays in registers (no dynamically indexed private array).
+__device__ __forceinline__ void lu_solve(const double* __restrict__ LU, int stride,
+                                         const int (&ipvt)[kNeqs], double (&x)[kNeqs]) {
+#pragma unroll
+    for (int k = 0; k < kNeqs - 1; ++k) {
+        const int p = ipvt[k];
+        double t = x[k];
+#pragma unroll
+        for (int i = k + 1; i < kNeqs; ++i) t = (p == i) ? x[i] : t;
+        const double xk = x[k];
+#pragma unroll
+        for (int i = k + 1; i < kNeqs; ++i) x[i] = (p == i) ? xk : x[i];
+        x[k] = t;
+#pragma unroll
+        for (int j = k + 1; j < kNeqs; ++j) x[j] += t * LU[(j * kNeqs + k) * stride];
+    }
+#pragma unroll
+    for (int kb = 0; kb < kNeqs; ++kb) {
+        const int k = kNeqs - 1 - kb;
+        x[k] /= LU[(k * kNeqs + k) * stride];
+        const double t = -x[k];
+#pragma unroll
+        for (int j = 0; j < k; ++j) x[j] += t * LU[(j * kNeqs + k) * stride];
+    }
+}
+
+// ---- ROS2S (reproducer.cpp:7066-7347) --------------------------------------------
+struct Ros2sCoef {
+    static constexpr double gamma = 0.292893218813452;
+    static constexpr double a21 = 2.0000000000000036;
+    static constexpr double a31 = 6.828427124746214;
+    static constexpr double a32 = 3.4142135623731007;
+    static constexpr double c21 = -6.828427124746214;
+    static constexpr double c31 = -10.949747468305889;
+    static constexpr double c32 = -7.535533905932761;
+    static constexpr double b1 = 6.828427124746214;
+    static constexpr double b2 = 3.414213562373101;
+    static constexpr double b3 = 1.0;
+    static constexpr double e1 = -0.23570226039551292;
+    static constexpr double e2 = -0.23570226039551567;
+    static constexpr double e3 = -0.13807118745769906;
+};
+constexpr double kUround = 1.0e-16, kFacMin = 0.2, kFacMax = 6.0, kSafe = 0.9;
+
+// Collapse tolerances (configure_ros2s, reproducer.cpp:7409-7412, 7582-7590).
+__device__ __forceinline__ constexpr double rtol_of(int i) { return i < kNumSpec ? 1.0e-4 : 1.0e-6; }
+__device__ __forceinline__ constexpr double atol_of(int i) { return i < kNumSpec ? 1.0e-4 : 1.0e-6; }
+
+struct Counters {
+    int n_step, n_rhs, n_jac, n_decomp, n_solve, n_accept, n_reject;
+};
+
+// RODAS<PrimordialChem>::integrate from t = 0 to tout (reproducer.cpp:7205-7347).
+// J and LU point at this cell's SoA workspace columns.  The RHS at the start
+// of an attempt, f(y), depends only on y, so it is evaluated together with
+// the Jacobian and reused by every retry of the same step: the reference
+// recomputes it per retry and gets the same bits.
+__device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, double dt0,
+                                               int max_steps, Counters& c,
+                                               double* __restrict__ J, double* __restrict__ LU,
+                                               int stride, double& t_end, double& dt_end) {
+    using C = Ros2sCoef;
+    c = Counters{0, 0, 0, 0, 0, 0, 0};
+    if (tout <= 0.0) {
+        t_end = 0.0;
+        dt_end = dt0;
+        return kBadInputs;
+    }
+    const double hmaxn = tout - 0.0;
+    double h = dt0;
+    if (fabs(h) <= 10.0 * kUround) h = 1.0e-6;
+    h = rmin(fabs(h), hmaxn);
+
+    bool reject = false, last = false;
+    int nsing = 0, n_step = 0, n_accept = 0;
+    double hacc = 0.0, erracc = 1.0, hopt = h, x = 0.0;
+    double f0[kNeqs];
+    int ipvt[kNeqs];
+
+    for (;;) {
+        if (n_step > max_steps) {
+            t_end = x;
+            dt_end = h;
+            return kTooManySteps;
+        }
+        if (0.1 * fabs(h) <= fabs(x) * kUround) {
+            t_end = x;
+            dt_end = h;
+            return kDtUnderflow;
+        }
+        if (last) {
+            t_end = x;
+            dt_end = hopt;
+            return kSuccess;
+        }
+        hopt = h;
+        if (x + h * 1.0001 >= tout) {
+            h = tout - x;
+            last = true;
+        }
+        jac_rhs(y, f0, J, stride);
+        c.n_jac += 1;
+
+        for (;;) {
+            const double fac = 1.0 / (h * C::gamma);
+            // E = -J + fac I into the LU workspace
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+#pragma unroll
+                for (int j = 0; j < kNeqs; ++j) {
+                    const double v = -J[(i * kNeqs + j) * stride];
+                    LU[(i * kNeqs + j) * stride] = (i == j) ? v + fac : v;
+                }
+            }
+            if (lu_factor(LU, stride, ipvt) != 0) {
+                nsing += 1;
+                if (nsing >= 5) {
+                    t_end = x;
+                    dt_end = h;
+                    return kLuError;
+                }
+                h *= 0.5;
+                reject = true;
+                last = false;
+                continue;
+            }
+            c.n_decomp += 1;
+
+            double ak1[kNeqs], ak2[kNeqs], ak3[kNeqs], ynew[kNeqs], dy[kNeqs];
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) ak1[i] = f0[i];
+            lu_solve(LU, stride, ipvt, ak1);
+            c.n_solve += 1;
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+                ynew[i] = y[i] + C::a21 * ak1[i];
+                ak2[i] = (C::c21 / h) * ak1[i];
+            }
+            rhs(ynew, dy);
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) ak2[i] += dy[i];
+            lu_solve(LU, stride, ipvt, ak2);
+            c.n_solve += 1;
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+                ynew[i] = y[i] + C::a31 * ak1[i] + C::a32 * ak2[i];
+                ak3[i] = (C::c31 * ak1[i] + C::c32 * ak2[i]) / h;
+            }
+            rhs(ynew, dy);
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) ak3[i] += dy[i];
+            lu_solve(LU, stride, ipvt, ak3);
+            c.n_solve += 1;
+            double err = 0.0;
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+                ynew[i] = y[i] + C::b1 * ak1[i] + C::b2 * ak2[i] + C::b3 * ak3[i];
+                const double w = C::e1 * ak1[i] + C::e2 * ak2[i] + C::e3 * ak3[i];
+                // error_norm (reproducer.cpp:7194-7202)
+                const double sk = atol_of(i) + rtol_of(i) * rmax(fabs(y[i]), fabs(ynew[i]));
+                const double term = w / sk;
+                err += term * term;
+            }
+            c.n_rhs += 3;
+            n_step += 1;
+            c.n_step = n_step;
+            err = sqrt(err / static_cast<double>(kNeqs));
+
+            const double raw_fac = cbrt(err) / kSafe;
+            const double fac_step = rmax(1.0 / kFacMax, rmin(1.0 / kFacMin, raw_fac));
+            double hnew = h / fac_step;
+            if (err <= 1.0) {
+                n_accept += 1;
+                c.n_accept = n_accept;
+                if (n_accept > 1) {
+                    const double facgus =
+                        rmax(1.0 / kFacMax,
+                             rmin(1.0 / kFacMin, (hacc / h) * cbrt((err * err) / erracc) / kSafe));
+                    hnew = h / rmax(fac_step, facgus);
+                }
+                hacc = h;
+                erracc = rmax(1.0e-2, err);
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) y[i] = ynew[i];
+                x += h;
+                if (fabs(hnew) > hmaxn) hnew = hmaxn;
+                if (reject) hnew = rmin(fabs(hnew), fabs(h));
+                reject = false;
+                h = hnew;
+                break;
+            }
+            reject = true;
+            last = false;
+            h = hnew;
+            if (n_accept >= 1) c.n_reject += 1;
+        }
+    }
+}
+
+}  // namespace wr

@@ -1,0 +1,938 @@
+// gfx950 kernels of the gridwide-independent collapse driver and the host-side
+// context that runs them.  Replaces run_cells_cuda and its kernels
+// (reproducer.cpp:7687-7975).
+//
+// Outer step (one HIP stream, no host round trip inside a step):
+//   k_prepare  one thread per cell: perturbation, free-fall dt candidate,
+//              terminal update, consistency checks (reproducer.cpp:7696-7734);
+//              each block reduces its candidates to a (dt, lowest cell) pair.
+//   k_plan     one block: grid-wide (dt, cell) minimum over the block pairs
+//              (the host loop at 7888-7896, same lowest-index tie break),
+//              next_grid_time and the termination tests (7894-7901).
+//   k_advance  one thread per cell: common-dt density update, stop tests and
+//              the cell's full adaptive ROS2S advance (7736-7785).
+//   k_commit   one thread: failure / zero-integrated termination, grid time
+//              and global step bookkeeping (7943-7958).
+// Every kernel returns immediately once the device control block says the
+// run is done, so the host enqueues several steps ahead and polls one flag.
+#include "wr_device.hpp"
+
+#include <float.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/warp_reactor.h"
+
+namespace wr {
+
+// SoA cell state on the device.
+struct DevCells {
+    double* xn;       // [14][cap]
+    double* rho;      // [cap]
+    double* T;
+    double* e;
+    double* time;
+    double* driver;
+    int* completed;
+    unsigned long long* stats;  // [7][cap]
+    int n;
+    int cap;
+};
+
+struct DevWs {
+    double* J;   // [225][cap]
+    double* LU;  // [225][cap]
+    int cap;
+};
+
+// Device control block of one run.
+struct Ctl {
+    double grid_time;
+    double dt_grid;
+    double next_grid_time;
+    unsigned long long cell_advances;
+    int failure;
+    int integrated;
+    int completed;
+    int step;
+    int max_steps;
+    int done;
+    int perturb;
+    int stop_reason;
+    int failed_step;
+    int failed_phase;
+    int limiting_cell;
+    int pad_;
+};
+
+constexpr int kPrepBlock = 256;
+constexpr int kAdvBlock = 64;
+constexpr int kPlanBlock = 1024;
+
+__device__ __forceinline__ void set_failure(Ctl* ctl, int code) {
+    atomicCAS(&ctl->failure, kSuccess, code);
+}
+
+// (dt, cell) pair order: smaller dt, then smaller cell.
+__device__ __forceinline__ bool pair_less(double da, int ca, double db, int cb) {
+    return da < db || (da == db && ca < cb);
+}
+
+__device__ __forceinline__ void block_min_pair(double& dt, int& cell, double* sdt, int* scell) {
+    // wave-level then block-level reduction through LDS
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double odt = __shfl_down(dt, off, 64);
+        const int oc = __shfl_down(cell, off, 64);
+        if (pair_less(odt, oc, dt, cell)) {
+            dt = odt;
+            cell = oc;
+        }
+    }
+    if (lane == 0) {
+        sdt[wave] = dt;
+        scell[wave] = cell;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; ++w) {
+            if (pair_less(sdt[w], scell[w], dt, cell)) {
+                dt = sdt[w];
+                cell = scell[w];
+            }
+        }
+    }
+}
+
+// ---- AoS <-> SoA ------------------------------------------------------------------
+__global__ void k_scatter(const wr_collapse_state* __restrict__ in, DevCells s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= s.n) return;
+    const wr_collapse_state& a = in[c];
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = a.current.xn[n];
+    s.rho[c] = a.current.rho;
+    s.T[c] = a.current.T;
+    s.e[c] = a.current.e;
+    s.time[c] = a.time;
+    s.driver[c] = a.density_driver;
+    s.completed[c] = a.completed_steps;
+    const unsigned long long* st = reinterpret_cast<const unsigned long long*>(&a.stats);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) s.stats[k * s.cap + c] = st[k];
+}
+
+__global__ void k_gather(wr_collapse_state* __restrict__ out, DevCells s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= s.n) return;
+    wr_collapse_state& a = out[c];
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) a.current.xn[n] = s.xn[n * s.cap + c];
+    a.current.rho = s.rho[c];
+    a.current.T = s.T[c];
+    a.current.e = s.e[c];
+    a.time = s.time[c];
+    a.density_driver = s.driver[c];
+    a.completed_steps = s.completed[c];
+    a.reserved_ = 0;
+    unsigned long long* st = reinterpret_cast<unsigned long long*>(&a.stats);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) st[k] = s.stats[k * s.cap + c];
+}
+
+// ---- outer step kernels ---------------------------------------------------------------
+
+// prepare_grid_timestep_kernel (reproducer.cpp:7696-7734) + per-block (dt, cell) min.
+__global__ __launch_bounds__(kPrepBlock) void k_prepare(DevCells s, Ctl* ctl, double* part_dt,
+                                                        int* part_cell) {
+    __shared__ double sdt[kPrepBlock / 64];
+    __shared__ int scell[kPrepBlock / 64];
+    if (ctl->done) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double cand = DBL_MAX;
+    int cand_cell = 0x7fffffff;
+    const int completed = ctl->completed;
+    if (c < s.n && ctl->failure == kSuccess) {
+        const int done_steps = s.completed[c];
+        if (done_steps >= completed) {
+            if (done_steps != completed || s.time[c] != ctl->grid_time) {
+                set_failure(ctl, kBadInputs);
+            } else {
+                double xn[kNumSpec];
+#pragma unroll
+                for (int n = 0; n < kNumSpec; ++n) xn[n] = s.xn[n * s.cap + c];
+                double driver = s.driver[c];
+                const int step = ctl->step;
+                // apply_perturbation (reproducer.cpp:7564-7580)
+                if (ctl->perturb && step != 0 && step % 20 == 0) {
+                    const double f = perturbation_factor(c, step);
+                    driver *= f;
+                    double rho = s.rho[c] * f;
+#pragma unroll
+                    for (int n = 0; n < kNumSpec; ++n) xn[n] *= f;
+                    const double T = post_step(xn, rho, s.e[c]);
+#pragma unroll
+                    for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
+                    s.rho[c] = rho;
+                    s.T[c] = T;
+                }
+                const double dt = collapse_timestep(xn);
+                if (!valid_positive(dt) || !valid_positive(driver)) {
+                    set_failure(ctl, kBadInputs);
+                } else if (dt < 10.0) {
+                    const double tff = dt / 1.0e-1;
+                    driver += dt * (driver / tff);
+                } else {
+                    cand = dt;
+                    cand_cell = c;
+                }
+                s.driver[c] = driver;
+            }
+        }
+    }
+    block_min_pair(cand, cand_cell, sdt, scell);
+    if (threadIdx.x == 0) {
+        part_dt[blockIdx.x] = cand;
+        part_cell[blockIdx.x] = cand_cell;
+    }
+}
+
+// grid-wide minimum and the host-loop checks of reproducer.cpp:7881-7901.
+__global__ __launch_bounds__(kPlanBlock) void k_plan(Ctl* ctl, const double* part_dt,
+                                                     const int* part_cell, int nparts) {
+    __shared__ double sdt[kPlanBlock / 64];
+    __shared__ int scell[kPlanBlock / 64];
+    if (ctl->done) return;
+    double dt = DBL_MAX;
+    int cell = 0x7fffffff;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        if (pair_less(part_dt[i], part_cell[i], dt, cell)) {
+            dt = part_dt[i];
+            cell = part_cell[i];
+        }
+    }
+    block_min_pair(dt, cell, sdt, scell);
+    if (threadIdx.x != 0) return;
+    if (ctl->failure != kSuccess) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_FAILURE;
+        ctl->failed_step = ctl->step;
+        ctl->failed_phase = 1;
+        return;
+    }
+    if (dt == DBL_MAX) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_NO_CANDIDATE;
+        ctl->limiting_cell = -1;
+        return;
+    }
+    const double next = ctl->grid_time + dt;
+    if (!valid_positive(dt) || !isfinite(next)) {
+        ctl->failure = kBadInputs;
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_FAILURE;
+        ctl->failed_step = ctl->step;
+        ctl->failed_phase = 1;
+        return;
+    }
+    ctl->dt_grid = dt;
+    ctl->next_grid_time = next;
+    ctl->limiting_cell = cell;
+    ctl->integrated = 0;
+}
+
+// advance_collapse_gridwide_kernel (reproducer.cpp:7736-7785), one thread per cell.
+__global__ __launch_bounds__(kAdvBlock) void k_advance(DevCells s, DevWs ws, Ctl* ctl) {
+    if (ctl->done) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= s.n || ctl->failure != kSuccess) return;
+    const int completed = ctl->completed;
+    if (s.completed[c] != completed) return;
+    double xn[kNumSpec];
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) xn[n] = s.xn[n * s.cap + c];
+    const double local_dt = collapse_timestep(xn);
+    if (local_dt < 10.0) return;
+    const double dt_grid = ctl->dt_grid;
+    const double old_density = s.driver[c];
+    const double tff = local_dt / 1.0e-1;
+    const double driver = old_density + dt_grid * (old_density / tff);
+    s.driver[c] = driver;
+    if (!valid_positive(driver) || driver > 2.0e-6) return;
+
+    const double ratio = driver / old_density;
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) xn[n] *= ratio;
+    const double rho = s.rho[c] * ratio;
+    s.rho[c] = rho;
+
+    // burn_ros2s (reproducer.cpp:7592-7627): eos_rt, y = (xn, e), integrate.
+    const double e0 = eos_energy(xn, s.T[c]);
+    double y[kNeqs];
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) y[n] = xn[n];
+    y[kNumSpec] = e0;
+    Counters cnt;
+    double t_end, dt_end;
+    const int result = ros2s_integrate(y, dt_grid, dt_grid, 10000000, cnt, ws.J + c, ws.LU + c,
+                                       ws.cap, t_end, dt_end);
+    const int cv[7] = {cnt.n_step, cnt.n_rhs, cnt.n_jac, cnt.n_decomp, cnt.n_solve,
+                       cnt.n_accept, cnt.n_reject};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        s.stats[k * s.cap + c] += static_cast<unsigned long long>(cv[k] > 0 ? cv[k] : 0);
+    }
+    if (result != kSuccess) {
+        // burn_ros2s leaves the state untouched except e (eos_rt) on failure
+        s.e[c] = e0;
+#pragma unroll
+        for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
+        set_failure(ctl, result);
+        return;
+    }
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) xn[n] = y[n];
+    const double e = y[kNumSpec];
+    const double T = post_step(xn, rho, e);
+#pragma unroll
+    for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
+    s.e[c] = e;
+    s.T[c] = T;
+    s.time[c] = ctl->next_grid_time;
+    s.completed[c] = completed + 1;
+    atomicAdd(&ctl->integrated, 1);
+}
+
+__global__ void k_commit(Ctl* ctl) {
+    if (ctl->done) return;
+    if (ctl->failure != kSuccess) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_FAILURE;
+        ctl->failed_step = ctl->step;
+        ctl->failed_phase = 2;
+        return;
+    }
+    if (ctl->integrated == 0) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_NONE_INTEGRATED;
+        return;
+    }
+    ctl->cell_advances += static_cast<unsigned long long>(ctl->integrated);
+    ctl->grid_time = ctl->next_grid_time;
+    ctl->completed += 1;
+    ctl->step += 1;
+    if (ctl->step >= ctl->max_steps) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_STEP_LIMIT;
+    }
+}
+
+// ---- building-block kernels (verification entry points) ------------------------------
+__global__ void k_eval_rhs(const double* __restrict__ y, int n, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double yy[kNeqs], f[kNeqs];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) yy[k] = y[i * kNeqs + k];
+    rhs(yy, f);
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) out[i * kNeqs + k] = f[k];
+}
+
+__global__ void k_eval_jac(const double* __restrict__ y, int n, DevWs ws, double* __restrict__ jac,
+                           double* __restrict__ f_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double yy[kNeqs], f[kNeqs];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) yy[k] = y[i * kNeqs + k];
+    jac_rhs(yy, f, ws.J + i, ws.cap);
+    for (int k = 0; k < kNeqs * kNeqs; ++k) jac[i * kNeqs * kNeqs + k] = ws.J[k * ws.cap + i];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) f_out[i * kNeqs + k] = f[k];
+}
+
+__global__ void k_eval_lu(double* __restrict__ a, int n, DevWs ws, int* __restrict__ ipvt_out,
+                          int* __restrict__ info_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < kNeqs * kNeqs; ++k) ws.LU[k * ws.cap + i] = a[i * kNeqs * kNeqs + k];
+    int ipvt[kNeqs];
+    info_out[i] = lu_factor(ws.LU + i, ws.cap, ipvt);
+    for (int k = 0; k < kNeqs * kNeqs; ++k) a[i * kNeqs * kNeqs + k] = ws.LU[k * ws.cap + i];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) ipvt_out[i * kNeqs + k] = ipvt[k];
+}
+
+__global__ void k_eval_solve(const double* __restrict__ lu, const int* __restrict__ ipvt_in, int n,
+                             DevWs ws, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < kNeqs * kNeqs; ++k) ws.LU[k * ws.cap + i] = lu[i * kNeqs * kNeqs + k];
+    int ipvt[kNeqs];
+    double xx[kNeqs];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) {
+        ipvt[k] = ipvt_in[i * kNeqs + k];
+        xx[k] = x[i * kNeqs + k];
+    }
+    lu_solve(ws.LU + i, ws.cap, ipvt, xx);
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) x[i * kNeqs + k] = xx[k];
+}
+
+__global__ __launch_bounds__(kAdvBlock) void k_eval_integrate(double* __restrict__ y,
+                                                              const double* __restrict__ tout,
+                                                              const double* __restrict__ dt0, int n,
+                                                              int max_steps, DevWs ws,
+                                                              int* __restrict__ counters,
+                                                              double* __restrict__ t_end,
+                                                              double* __restrict__ dt_end,
+                                                              int* __restrict__ result) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double yy[kNeqs];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) yy[k] = y[i * kNeqs + k];
+    Counters c;
+    double te, de;
+    const int r = ros2s_integrate(yy, tout[i], dt0[i], max_steps, c, ws.J + i, ws.LU + i, ws.cap,
+                                  te, de);
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) y[i * kNeqs + k] = yy[k];
+    const int cv[7] = {c.n_step, c.n_rhs, c.n_jac, c.n_decomp, c.n_solve, c.n_accept, c.n_reject};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) counters[i * 7 + k] = cv[k];
+    t_end[i] = te;
+    dt_end[i] = de;
+    result[i] = r;
+}
+
+}  // namespace wr
+
+// =====================================================================================
+// Host side: context and C ABI
+// =====================================================================================
+using namespace wr;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+bool hip_ok(hipError_t st, const char* action, std::string* msg = nullptr) {
+    if (st == hipSuccess) return true;
+    const std::string m = std::string(action) + " failed: " + hipGetErrorString(st);
+    g_last_error = m;
+    if (msg) *msg = m;
+    return false;
+}
+
+#define WR_HIP(call, action)                                   \
+    do {                                                       \
+        if (!hip_ok((call), (action))) return WR_ERR_HIP;      \
+    } while (0)
+
+int blocks_for(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+struct wr_context {
+    int device = 0;
+    int cap = 0;
+    int n = 0;
+    hipStream_t stream = nullptr;
+    DevCells cells{};
+    DevWs ws{};
+    Ctl* ctl = nullptr;         // device
+    Ctl* ctl_host = nullptr;    // pinned mirror
+    double* part_dt = nullptr;  // per-prepare-block partials
+    int* part_cell = nullptr;
+    wr_collapse_state* staging = nullptr;  // device AoS staging buffer
+    void* cells_mem = nullptr;
+    void* ws_mem = nullptr;
+    // HIP-event timing of the advance launches of wr_run (when enabled)
+    static constexpr int kChunk = 8;
+    bool timing = false;
+    hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
+    double adv_ms = 0.0;
+    int adv_launches = 0;
+    wr_run_info info{};
+};
+
+extern "C" {
+
+int wr_abi_version(void) { return WR_ABI_VERSION; }
+
+const char* wr_last_error(void) { return g_last_error.c_str(); }
+
+void wr_context_destroy(wr_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->cells_mem);
+    (void)hipFree(ctx->ws_mem);
+    (void)hipFree(ctx->ctl);
+    (void)hipFree(ctx->part_dt);
+    (void)hipFree(ctx->part_cell);
+    (void)hipFree(ctx->staging);
+    if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
+    for (int k = 0; k < wr_context::kChunk; ++k) {
+        if (ctx->adv_start[k]) (void)hipEventDestroy(ctx->adv_start[k]);
+        if (ctx->adv_stop[k]) (void)hipEventDestroy(ctx->adv_stop[k]);
+    }
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int wr_context_create(int device, int capacity_cells, wr_context** out) {
+    if (!out || capacity_cells <= 0) return fail(WR_ERR_ARGUMENT, "wr_context_create: bad argument");
+    *out = nullptr;
+    auto* ctx = new wr_context();
+    ctx->device = device;
+    // keep every SoA column 256-byte aligned
+    ctx->cap = (capacity_cells + 63) / 64 * 64;
+    const size_t cap = static_cast<size_t>(ctx->cap);
+    auto cleanup = [&](int code) {
+        wr_context_destroy(ctx);
+        return code;
+    };
+    if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return cleanup(WR_ERR_HIP);
+    if (!hip_ok(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate"))
+        return cleanup(WR_ERR_HIP);
+    const size_t cell_bytes = cap * (sizeof(double) * (kNumSpec + 5) + sizeof(int) +
+                                     sizeof(unsigned long long) * 7) + 4096;
+    if (!hip_ok(hipMalloc(&ctx->cells_mem, cell_bytes), "hipMalloc(cells)"))
+        return cleanup(WR_ERR_HIP);
+    char* p = static_cast<char*>(ctx->cells_mem);
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    ctx->cells.xn = reinterpret_cast<double*>(take(cap * kNumSpec * sizeof(double)));
+    ctx->cells.rho = reinterpret_cast<double*>(take(cap * sizeof(double)));
+    ctx->cells.T = reinterpret_cast<double*>(take(cap * sizeof(double)));
+    ctx->cells.e = reinterpret_cast<double*>(take(cap * sizeof(double)));
+    ctx->cells.time = reinterpret_cast<double*>(take(cap * sizeof(double)));
+    ctx->cells.driver = reinterpret_cast<double*>(take(cap * sizeof(double)));
+    ctx->cells.stats = reinterpret_cast<unsigned long long*>(take(cap * 7 * sizeof(unsigned long long)));
+    ctx->cells.completed = reinterpret_cast<int*>(take(cap * sizeof(int)));
+    ctx->cells.cap = ctx->cap;
+    ctx->cells.n = 0;
+    const size_t ws_bytes = cap * kNeqs * kNeqs * sizeof(double) * 2;
+    if (!hip_ok(hipMalloc(&ctx->ws_mem, ws_bytes), "hipMalloc(workspace)")) return cleanup(WR_ERR_HIP);
+    ctx->ws.J = static_cast<double*>(ctx->ws_mem);
+    ctx->ws.LU = ctx->ws.J + cap * kNeqs * kNeqs;
+    ctx->ws.cap = ctx->cap;
+    const int nparts = blocks_for(ctx->cap, kPrepBlock);
+    if (!hip_ok(hipMalloc(&ctx->ctl, sizeof(Ctl)), "hipMalloc(ctl)") ||
+        !hip_ok(hipHostMalloc(&ctx->ctl_host, sizeof(Ctl)), "hipHostMalloc(ctl)") ||
+        !hip_ok(hipMalloc(&ctx->part_dt, nparts * sizeof(double)), "hipMalloc(partials)") ||
+        !hip_ok(hipMalloc(&ctx->part_cell, nparts * sizeof(int)), "hipMalloc(partials)") ||
+        !hip_ok(hipMalloc(&ctx->staging, cap * sizeof(wr_collapse_state)), "hipMalloc(staging)"))
+        return cleanup(WR_ERR_HIP);
+    for (int k = 0; k < wr_context::kChunk; ++k) {
+        if (!hip_ok(hipEventCreate(&ctx->adv_start[k]), "hipEventCreate") ||
+            !hip_ok(hipEventCreate(&ctx->adv_stop[k]), "hipEventCreate"))
+            return cleanup(WR_ERR_HIP);
+    }
+    memset(ctx->ctl_host, 0, sizeof(Ctl));
+    *out = ctx;
+    return WR_SUCCESS;
+}
+
+int wr_upload_cells(wr_context* ctx, const wr_collapse_state* cells, int num_cells) {
+    if (!ctx || (!cells && num_cells > 0) || num_cells < 0)
+        return fail(WR_ERR_ARGUMENT, "wr_upload_cells: bad argument");
+    if (num_cells > ctx->cap) return fail(WR_ERR_CAPACITY, "wr_upload_cells: more cells than capacity");
+    WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->n = num_cells;
+    ctx->cells.n = num_cells;
+    if (num_cells == 0) return WR_SUCCESS;
+    WR_HIP(hipMemcpyAsync(ctx->staging, cells, num_cells * sizeof(wr_collapse_state),
+                          hipMemcpyHostToDevice, ctx->stream),
+           "hipMemcpy(cells to device)");
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(num_cells, 256)), dim3(256), 0, ctx->stream,
+                       ctx->staging, ctx->cells);
+    WR_HIP(hipGetLastError(), "k_scatter launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "k_scatter synchronize");
+    return WR_SUCCESS;
+}
+
+int wr_download_cells(wr_context* ctx, wr_collapse_state* cells, int num_cells) {
+    if (!ctx || (!cells && num_cells > 0) || num_cells < 0 || num_cells > ctx->n)
+        return fail(WR_ERR_ARGUMENT, "wr_download_cells: bad argument");
+    if (num_cells == 0) return WR_SUCCESS;
+    WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(num_cells, 256)), dim3(256), 0, ctx->stream,
+                       ctx->staging, ctx->cells);
+    WR_HIP(hipGetLastError(), "k_gather launch");
+    WR_HIP(hipMemcpyAsync(cells, ctx->staging, num_cells * sizeof(wr_collapse_state),
+                          hipMemcpyDeviceToHost, ctx->stream),
+           "hipMemcpy(cells to host)");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return WR_SUCCESS;
+}
+
+static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time) {
+    Ctl h{};
+    h.grid_time = grid_time;
+    h.failure = kSuccess;
+    h.completed = completed;
+    h.step = 0;
+    h.max_steps = max_steps;
+    h.done = (max_steps <= 0) ? 1 : 0;
+    h.perturb = perturb ? 1 : 0;
+    h.stop_reason = (max_steps <= 0) ? WR_STOP_STEP_LIMIT : WR_STOP_NONE;
+    h.failed_step = -1;
+    h.limiting_cell = -1;
+    *ctx->ctl_host = h;
+    WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
+           "hipMemcpy(control to device)");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return WR_SUCCESS;
+}
+
+static int read_ctl(wr_context* ctx) {
+    WR_HIP(hipMemcpyAsync(ctx->ctl_host, ctx->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream),
+           "hipMemcpy(control to host)");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return WR_SUCCESS;
+}
+
+static int launch_prepare(wr_context* ctx) {
+    const int nb = blocks_for(ctx->n, kPrepBlock);
+    hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(kPrepBlock), 0, ctx->stream, ctx->cells, ctx->ctl,
+                       ctx->part_dt, ctx->part_cell);
+    WR_HIP(hipGetLastError(), "prepare_grid_timestep_kernel launch");
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanBlock), 0, ctx->stream, ctx->ctl, ctx->part_dt,
+                       ctx->part_cell, nb);
+    WR_HIP(hipGetLastError(), "grid timestep reduction launch");
+    return WR_SUCCESS;
+}
+
+static int launch_advance(wr_context* ctx) {
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
+                       ctx->stream, ctx->cells, ctx->ws, ctx->ctl);
+    WR_HIP(hipGetLastError(), "advance_collapse_gridwide_kernel launch");
+    return WR_SUCCESS;
+}
+
+static void fill_info(wr_context* ctx) {
+    const Ctl& h = *ctx->ctl_host;
+    wr_run_info& i = ctx->info;
+    i.status = h.failure;
+    i.stop_reason = h.stop_reason;
+    i.completed_global_steps = h.completed;
+    i.failed_step = h.failed_step;
+    i.failed_phase = h.failed_phase;
+    i.grid_time = h.grid_time;
+    i.cell_advances = h.cell_advances;
+}
+
+int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_steps,
+           double grid_time) {
+    if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_run: null context");
+    if (max_steps <= 0) max_steps = 1000;
+    WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    int st = init_ctl(ctx, perturb, ctx->n > 0 ? max_steps : 0, completed_global_steps, grid_time);
+    if (st != WR_SUCCESS) return st;
+    ctx->adv_ms = 0.0;
+    ctx->adv_launches = 0;
+    if (ctx->n == 0) {
+        ctx->ctl_host->stop_reason = WR_STOP_NO_CANDIDATE;
+        fill_info(ctx);
+        return WR_SUCCESS;
+    }
+    // Steps are enqueued in chunks; kernels of steps past the end return at once.
+    const int chunk = wr_context::kChunk;
+    for (int issued = 0; issued < max_steps;) {
+        const int todo = (max_steps - issued < chunk) ? max_steps - issued : chunk;
+        const int steps_before = ctx->ctl_host->step;
+        for (int k = 0; k < todo; ++k) {
+            if ((st = launch_prepare(ctx)) != WR_SUCCESS) return st;
+            if (ctx->timing) WR_HIP(hipEventRecord(ctx->adv_start[k], ctx->stream), "hipEventRecord");
+            if ((st = launch_advance(ctx)) != WR_SUCCESS) return st;
+            if (ctx->timing) WR_HIP(hipEventRecord(ctx->adv_stop[k], ctx->stream), "hipEventRecord");
+            hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
+            WR_HIP(hipGetLastError(), "commit launch");
+        }
+        issued += todo;
+        if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
+        if (ctx->timing) {
+            // advance launches that did work: completed steps of this chunk, plus
+            // the last one when the run ended in its advance phase
+            const Ctl& h = *ctx->ctl_host;
+            int ran = h.step - steps_before;
+            if (h.done && (h.stop_reason == WR_STOP_NONE_INTEGRATED ||
+                           (h.stop_reason == WR_STOP_FAILURE && h.failed_phase == 2)))
+                ran += 1;
+            for (int k = 0; k < ran && k < todo; ++k) {
+                float ms = 0.0f;
+                WR_HIP(hipEventElapsedTime(&ms, ctx->adv_start[k], ctx->adv_stop[k]),
+                       "hipEventElapsedTime");
+                ctx->adv_ms += ms;
+                ctx->adv_launches += 1;
+            }
+        }
+        if (ctx->ctl_host->done) break;
+    }
+    fill_info(ctx);
+    return ctx->ctl_host->failure;
+}
+
+int wr_set_timing(wr_context* ctx, int enable) {
+    if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_set_timing: null context");
+    ctx->timing = enable != 0;
+    return WR_SUCCESS;
+}
+
+int wr_run_info_get(const wr_context* ctx, wr_run_info* out) {
+    if (!ctx || !out) return fail(WR_ERR_ARGUMENT, "wr_run_info_get: bad argument");
+    *out = ctx->info;
+    return WR_SUCCESS;
+}
+
+int wr_last_advance_timing(const wr_context* ctx, double* total_ms, int* launches) {
+    if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_last_advance_timing: null context");
+    if (total_ms) *total_ms = ctx->adv_ms;
+    if (launches) *launches = ctx->adv_launches;
+    return WR_SUCCESS;
+}
+
+int wr_prepare(wr_context* ctx, int completed_global_steps, double grid_time, int step, int perturb,
+               double* dt_grid, int* limiting_cell) {
+    if (!ctx || !dt_grid) return fail(WR_ERR_ARGUMENT, "wr_prepare: bad argument");
+    WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    int st = init_ctl(ctx, perturb, step + 1, completed_global_steps, grid_time);
+    if (st != WR_SUCCESS) return st;
+    if (ctx->n == 0) {
+        *dt_grid = DBL_MAX;
+        if (limiting_cell) *limiting_cell = -1;
+        return WR_SUCCESS;
+    }
+    // the control block carries the step index used for the perturbation
+    ctx->ctl_host->step = step;
+    WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
+           "hipMemcpy(control to device)");
+    if ((st = launch_prepare(ctx)) != WR_SUCCESS) return st;
+    if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
+    const Ctl& h = *ctx->ctl_host;
+    if (h.failure != kSuccess) {
+        *dt_grid = DBL_MAX;
+        if (limiting_cell) *limiting_cell = -1;
+        return h.failure;
+    }
+    if (h.stop_reason == WR_STOP_NO_CANDIDATE) {
+        *dt_grid = DBL_MAX;
+        if (limiting_cell) *limiting_cell = -1;
+        return WR_SUCCESS;
+    }
+    *dt_grid = h.dt_grid;
+    if (limiting_cell) *limiting_cell = h.limiting_cell;
+    return WR_SUCCESS;
+}
+
+int wr_advance(wr_context* ctx, int completed_global_steps, double next_grid_time, double dt_grid,
+               int* integrated) {
+    if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_advance: null context");
+    WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    int st = init_ctl(ctx, 0, 1, completed_global_steps, 0.0);
+    if (st != WR_SUCCESS) return st;
+    ctx->ctl_host->dt_grid = dt_grid;
+    ctx->ctl_host->next_grid_time = next_grid_time;
+    ctx->ctl_host->integrated = 0;
+    WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
+           "hipMemcpy(control to device)");
+    if (ctx->n > 0 && (st = launch_advance(ctx)) != WR_SUCCESS) return st;
+    if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
+    if (integrated) *integrated = ctx->ctl_host->integrated;
+    return ctx->ctl_host->failure;
+}
+
+// Drop-in for run_cells_cuda (reproducer.cpp:7791-7975).
+int wr_run_cells(wr_collapse_state* cells, int num_cells, int perturb, int* completed_global_steps) {
+    if (!cells || num_cells <= 0 || !completed_global_steps) {
+        fprintf(stderr, "wr_run_cells: bad argument\n");
+        return WR_BAD_INPUTS;
+    }
+    wr_context* ctx = nullptr;
+    int st = wr_context_create(0, num_cells, &ctx);
+    if (st == WR_SUCCESS) st = wr_upload_cells(ctx, cells, num_cells);
+    if (st != WR_SUCCESS) {
+        fprintf(stderr, "%s\n", g_last_error.c_str());
+        wr_context_destroy(ctx);
+        return WR_BAD_INPUTS;
+    }
+    st = wr_run(ctx, perturb, 1000, *completed_global_steps, 0.0);
+    int result = WR_SUCCESS;
+    if (st <= WR_ERR_HIP) {
+        fprintf(stderr, "%s\n", g_last_error.c_str());
+        result = WR_BAD_INPUTS;
+    } else {
+        const wr_run_info& i = ctx->info;
+        *completed_global_steps = i.completed_global_steps;
+        if (i.status != WR_SUCCESS) {
+            result = i.status;
+            if (i.failed_phase == 1)
+                fprintf(stderr, "grid timestep preparation failed on hip collapse step %d with code %d\n",
+                        i.failed_step, i.status);
+            else
+                fprintf(stderr, "ROS2S failed on hip collapse step %d with code %d\n", i.failed_step,
+                        i.status);
+        }
+    }
+    const int dl = wr_download_cells(ctx, cells, num_cells);
+    if (dl != WR_SUCCESS) {
+        fprintf(stderr, "%s\n", g_last_error.c_str());
+        if (result == WR_SUCCESS) result = WR_BAD_INPUTS;
+    }
+    wr_context_destroy(ctx);
+    return result;
+}
+
+// ---- batched verification entry points ---------------------------------------------------
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int eval_common(wr_context* ctx, int n) {
+    if (!ctx || n < 0) return fail(WR_ERR_ARGUMENT, "eval: bad argument");
+    if (n > ctx->cap) return fail(WR_ERR_CAPACITY, "eval: more items than context capacity");
+    WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    return WR_SUCCESS;
+}
+
+}  // namespace
+
+int wr_eval_rhs(wr_context* ctx, const double* y, int n, double* dydt) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    DevBuf dy, dout;
+    const size_t b = static_cast<size_t>(n) * kNeqs * sizeof(double);
+    WR_HIP(hipMalloc(&dy.p, b), "hipMalloc");
+    WR_HIP(hipMalloc(&dout.p, b), "hipMalloc");
+    WR_HIP(hipMemcpy(dy.p, y, b, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_rhs, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream,
+                       static_cast<double*>(dy.p), n, static_cast<double*>(dout.p));
+    WR_HIP(hipGetLastError(), "k_eval_rhs launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(dydt, dout.p, b, hipMemcpyDeviceToHost), "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+int wr_eval_jac(wr_context* ctx, const double* y, int n, double* jac) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    DevBuf dy, dj, df;
+    const size_t b = static_cast<size_t>(n) * kNeqs * sizeof(double);
+    WR_HIP(hipMalloc(&dy.p, b), "hipMalloc");
+    WR_HIP(hipMalloc(&df.p, b), "hipMalloc");
+    WR_HIP(hipMalloc(&dj.p, b * kNeqs), "hipMalloc");
+    WR_HIP(hipMemcpy(dy.p, y, b, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_jac, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream,
+                       static_cast<double*>(dy.p), n, ctx->ws, static_cast<double*>(dj.p),
+                       static_cast<double*>(df.p));
+    WR_HIP(hipGetLastError(), "k_eval_jac launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(jac, dj.p, b * kNeqs, hipMemcpyDeviceToHost), "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+int wr_eval_lu(wr_context* ctx, double* a, int n, int32_t* ipvt, int32_t* info) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    DevBuf da, dp, di;
+    const size_t b = static_cast<size_t>(n) * kNeqs * kNeqs * sizeof(double);
+    WR_HIP(hipMalloc(&da.p, b), "hipMalloc");
+    WR_HIP(hipMalloc(&dp.p, static_cast<size_t>(n) * kNeqs * sizeof(int)), "hipMalloc");
+    WR_HIP(hipMalloc(&di.p, static_cast<size_t>(n) * sizeof(int)), "hipMalloc");
+    WR_HIP(hipMemcpy(da.p, a, b, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_lu, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream,
+                       static_cast<double*>(da.p), n, ctx->ws, static_cast<int*>(dp.p),
+                       static_cast<int*>(di.p));
+    WR_HIP(hipGetLastError(), "k_eval_lu launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(a, da.p, b, hipMemcpyDeviceToHost), "hipMemcpy");
+    WR_HIP(hipMemcpy(ipvt, dp.p, static_cast<size_t>(n) * kNeqs * sizeof(int), hipMemcpyDeviceToHost),
+           "hipMemcpy");
+    WR_HIP(hipMemcpy(info, di.p, static_cast<size_t>(n) * sizeof(int), hipMemcpyDeviceToHost),
+           "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+int wr_eval_lu_solve(wr_context* ctx, const double* lu, const int32_t* ipvt, int n, double* x) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    DevBuf dl, dp, dx;
+    const size_t b = static_cast<size_t>(n) * kNeqs * kNeqs * sizeof(double);
+    const size_t bx = static_cast<size_t>(n) * kNeqs * sizeof(double);
+    WR_HIP(hipMalloc(&dl.p, b), "hipMalloc");
+    WR_HIP(hipMalloc(&dp.p, static_cast<size_t>(n) * kNeqs * sizeof(int)), "hipMalloc");
+    WR_HIP(hipMalloc(&dx.p, bx), "hipMalloc");
+    WR_HIP(hipMemcpy(dl.p, lu, b, hipMemcpyHostToDevice), "hipMemcpy");
+    WR_HIP(hipMemcpy(dp.p, ipvt, static_cast<size_t>(n) * kNeqs * sizeof(int), hipMemcpyHostToDevice),
+           "hipMemcpy");
+    WR_HIP(hipMemcpy(dx.p, x, bx, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_solve, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream,
+                       static_cast<double*>(dl.p), static_cast<int*>(dp.p), n, ctx->ws,
+                       static_cast<double*>(dx.p));
+    WR_HIP(hipGetLastError(), "k_eval_solve launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(x, dx.p, bx, hipMemcpyDeviceToHost), "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+int wr_eval_integrate(wr_context* ctx, double* y, const double* tout, const double* dt0, int n,
+                      int max_steps, int32_t* counters, double* t_end, double* dt_end,
+                      int32_t* result) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    if (max_steps <= 0) max_steps = 10000000;
+    DevBuf dy, dto, ddt, dc, dte, dde, dr;
+    const size_t bn = static_cast<size_t>(n) * sizeof(double);
+    WR_HIP(hipMalloc(&dy.p, bn * kNeqs), "hipMalloc");
+    WR_HIP(hipMalloc(&dto.p, bn), "hipMalloc");
+    WR_HIP(hipMalloc(&ddt.p, bn), "hipMalloc");
+    WR_HIP(hipMalloc(&dc.p, static_cast<size_t>(n) * 7 * sizeof(int)), "hipMalloc");
+    WR_HIP(hipMalloc(&dte.p, bn), "hipMalloc");
+    WR_HIP(hipMalloc(&dde.p, bn), "hipMalloc");
+    WR_HIP(hipMalloc(&dr.p, static_cast<size_t>(n) * sizeof(int)), "hipMalloc");
+    WR_HIP(hipMemcpy(dy.p, y, bn * kNeqs, hipMemcpyHostToDevice), "hipMemcpy");
+    WR_HIP(hipMemcpy(dto.p, tout, bn, hipMemcpyHostToDevice), "hipMemcpy");
+    WR_HIP(hipMemcpy(ddt.p, dt0, bn, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_integrate, dim3(blocks_for(n, kAdvBlock)), dim3(kAdvBlock), 0,
+                       ctx->stream, static_cast<double*>(dy.p), static_cast<double*>(dto.p),
+                       static_cast<double*>(ddt.p), n, max_steps, ctx->ws, static_cast<int*>(dc.p),
+                       static_cast<double*>(dte.p), static_cast<double*>(dde.p),
+                       static_cast<int*>(dr.p));
+    WR_HIP(hipGetLastError(), "k_eval_integrate launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(y, dy.p, bn * kNeqs, hipMemcpyDeviceToHost), "hipMemcpy");
+    WR_HIP(hipMemcpy(counters, dc.p, static_cast<size_t>(n) * 7 * sizeof(int), hipMemcpyDeviceToHost),
+           "hipMemcpy");
+    WR_HIP(hipMemcpy(t_end, dte.p, bn, hipMemcpyDeviceToHost), "hipMemcpy");
+    WR_HIP(hipMemcpy(dt_end, dde.p, bn, hipMemcpyDeviceToHost), "hipMemcpy");
+    WR_HIP(hipMemcpy(result, dr.p, static_cast<size_t>(n) * sizeof(int), hipMemcpyDeviceToHost),
+           "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+"""ctypes binding of libwarp_reactor.so (include/warp_reactor.h).
+
+Cell arrays are numpy structured arrays with the 216-byte CollapseState
+layout (reproducer.cpp:7455-7461); packed final states use the 172-byte
+PackedFinalState layout (reproducer.cpp:7488-7505).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+
+NUM_SPEC = 14
+NEQS = 15
+SPECIES_NAMES = ("E", "Hp", "H", "Hm", "Dp", "D", "H2p", "Dm", "H2", "HDp", "HD",
+                 "HEpp", "HEp", "HE")
+STATS_FIELDS = ("internal_steps", "rhs_calls", "jacobian_calls", "decompositions",
+                "linear_solves", "accepted_steps", "rejected_steps")
+
+BURN_DTYPE = np.dtype([("rho", "<f8"), ("T", "<f8"), ("e", "<f8"), ("xn", "<f8", (NUM_SPEC,))])
+STATS_DTYPE = np.dtype([(f, "<u8") for f in STATS_FIELDS])
+CELL_DTYPE = np.dtype([("current", BURN_DTYPE), ("time", "<f8"), ("density_driver", "<f8"),
+                       ("completed_steps", "<i4"), ("reserved_", "<i4"), ("stats", STATS_DTYPE)])
+PACKED_DTYPE = np.dtype([("cell", "<i4"), ("i", "<i4"), ("j", "<i4"), ("k", "<i4"),
+                         ("completed_steps", "<i4"), ("time", "<f8"), ("density_driver", "<f8"),
+                         ("rho", "<f8"), ("T", "<f8"), ("e", "<f8"), ("xn", "<f8", (NUM_SPEC,))])
+assert CELL_DTYPE.itemsize == 216 and PACKED_DTYPE.itemsize == 172
+
+SUCCESS, BAD_INPUTS, DT_UNDERFLOW, TOO_MANY_STEPS = 1, -1, -2, -4
+TOO_MUCH_ACCURACY_REQUESTED, LU_DECOMPOSITION_ERROR = -5, -7
+ERR_HIP, ERR_ARGUMENT, ERR_CAPACITY, ERR_IO = -100, -101, -102, -103
+
+
+class WarpReactorError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{message} (status {code})")
+        self.code = code
+
+
+class RunInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("stop_reason", ctypes.c_int32),
+                ("completed_global_steps", ctypes.c_int32), ("failed_step", ctypes.c_int32),
+                ("failed_phase", ctypes.c_int32), ("reserved_", ctypes.c_int32),
+                ("grid_time", ctypes.c_double), ("cell_advances", ctypes.c_uint64)]
+
+
+class ComparisonSummary(ctypes.Structure):
+    _fields_ = [("pass_", ctypes.c_int32), ("failures", ctypes.c_int32),
+                ("max_deuterium_species_rel_error", ctypes.c_double),
+                ("max_non_deuterium_species_rel_error", ctypes.c_double),
+                ("max_thermodynamic_rel_error", ctypes.c_double),
+                ("max_rho_rel_error", ctypes.c_double),
+                ("first_failure", ctypes.c_char * 512)]
+
+
+def library_path():
+    return os.environ.get("WR_LIBRARY", os.path.join(PKG, "lib", "libwarp_reactor.so"))
+
+
+_LIB = None
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SIGS = {
+    "wr_abi_version": (_I, []),
+    "wr_last_error": (ctypes.c_char_p, []),
+    "wr_run_cells": (_I, [_VP, _I, _I, ctypes.POINTER(_I)]),
+    "wr_context_create": (_I, [_I, _I, ctypes.POINTER(_VP)]),
+    "wr_context_destroy": (None, [_VP]),
+    "wr_upload_cells": (_I, [_VP, _VP, _I]),
+    "wr_download_cells": (_I, [_VP, _VP, _I]),
+    "wr_run": (_I, [_VP, _I, _I, _I, _D]),
+    "wr_run_info_get": (_I, [_VP, ctypes.POINTER(RunInfo)]),
+    "wr_prepare": (_I, [_VP, _I, _D, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
+    "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
+    "wr_set_timing": (_I, [_VP, _I]),
+    "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
+    "wr_eval_rhs": (_I, [_VP, _VP, _I, _VP]),
+    "wr_eval_jac": (_I, [_VP, _VP, _I, _VP]),
+    "wr_eval_lu": (_I, [_VP, _VP, _I, _VP, _VP]),
+    "wr_eval_lu_solve": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "wr_eval_integrate": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+    "wr_make_collapse_states": (_I, [_VP, _I]),
+    "wr_pack_final_states": (_I, [_VP, _I, _I, _VP]),
+    "wr_write_final_states": (_I, [_VP, _I, _I, ctypes.c_char_p]),
+    "wr_read_final_states": (_I, [ctypes.c_char_p, _I, _VP]),
+    "wr_compare_final_states": (_I, [_VP, _I, _I, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.POINTER(ComparisonSummary)]),
+}
+EXPORTED = tuple(_SIGS)
+
+
+def library():
+    """Load libwarp_reactor.so once; raises if it is missing (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise WarpReactorError(ERR_ARGUMENT, f"HIP library not built: {path} "
+                                   "(run python warp-reactor_amd/build.py)")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_VP)
+
+
+def _check(code, what):
+    if code != SUCCESS:
+        msg = library().wr_last_error().decode(errors="replace")
+        raise WarpReactorError(code, f"{what}: {msg}")
+
+
+def make_collapse_states(n):
+    cells = np.zeros(n, dtype=CELL_DTYPE)
+    _check(library().wr_make_collapse_states(_ptr(cells), n), "wr_make_collapse_states")
+    return cells
+
+
+def run_cells(cells, perturb=True):
+    """run_cells_cuda (reproducer.cpp:7791): advances `cells` in place.
+
+    Returns (IntegratorResult code, completed_global_steps)."""
+    assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
+    completed = ctypes.c_int(0)
+    r = library().wr_run_cells(_ptr(cells), len(cells), int(bool(perturb)), ctypes.byref(completed))
+    return r, completed.value
+
+
+def pack_final_states(cells, grid_dim):
+    out = np.zeros(len(cells), dtype=PACKED_DTYPE)
+    _check(library().wr_pack_final_states(_ptr(cells), len(cells), grid_dim, _ptr(out)),
+           "wr_pack_final_states")
+    return out
+
+
+def write_final_states(cells, grid_dim, path):
+    _check(library().wr_write_final_states(_ptr(cells), len(cells), grid_dim,
+                                           os.fsencode(path)), "wr_write_final_states")
+
+
+def read_final_states(path, expected):
+    out = np.zeros(expected, dtype=PACKED_DTYPE)
+    _check(library().wr_read_final_states(os.fsencode(path), expected, _ptr(out)),
+           "wr_read_final_states")
+    return out
+
+
+def compare_final_states(cells, grid_dim, path, csv_path=None):
+    s = ComparisonSummary()
+    _check(library().wr_compare_final_states(
+        _ptr(cells), len(cells), grid_dim, os.fsencode(path),
+        os.fsencode(csv_path) if csv_path else None, ctypes.byref(s)), "wr_compare_final_states")
+    return {"pass": bool(s.pass_), "failures": s.failures,
+            "max_deuterium_species_rel_error": s.max_deuterium_species_rel_error,
+            "max_non_deuterium_species_rel_error": s.max_non_deuterium_species_rel_error,
+            "max_thermodynamic_rel_error": s.max_thermodynamic_rel_error,
+            "max_rho_rel_error": s.max_rho_rel_error,
+            "first_failure": s.first_failure.decode()}
+
+
+class Context:
+    """A device context holding up to `capacity` cells (SoA on the GPU)."""
+
+    def __init__(self, capacity, device=0):
+        self._lib = library()
+        h = _VP()
+        _check(self._lib.wr_context_create(device, capacity, ctypes.byref(h)), "wr_context_create")
+        self._h = h
+        self.capacity = capacity
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.wr_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, cells):
+        assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
+        _check(self._lib.wr_upload_cells(self._h, _ptr(cells), len(cells)), "wr_upload_cells")
+
+    def download(self, cells):
+        assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
+        _check(self._lib.wr_download_cells(self._h, _ptr(cells), len(cells)), "wr_download_cells")
+
+    def run(self, perturb=True, max_steps=1000, completed=0, grid_time=0.0):
+        """Outer loop on resident cells; returns the run info as a dict."""
+        r = self._lib.wr_run(self._h, int(bool(perturb)), max_steps, completed, grid_time)
+        if r <= ERR_HIP:
+            _check(r, "wr_run")
+        info = RunInfo()
+        _check(self._lib.wr_run_info_get(self._h, ctypes.byref(info)), "wr_run_info_get")
+        return {f: getattr(info, f) for f, _ in RunInfo._fields_ if f != "reserved_"}
+
+    def set_timing(self, enable=True):
+        _check(self._lib.wr_set_timing(self._h, int(bool(enable))), "wr_set_timing")
+
+    def advance_timing(self):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _check(self._lib.wr_last_advance_timing(self._h, ctypes.byref(ms), ctypes.byref(n)),
+               "wr_last_advance_timing")
+        return ms.value, n.value
+
+    def prepare(self, completed, grid_time, step, perturb=True):
+        dt, cell = ctypes.c_double(), ctypes.c_int()
+        r = self._lib.wr_prepare(self._h, completed, grid_time, step, int(bool(perturb)),
+                                 ctypes.byref(dt), ctypes.byref(cell))
+        if r <= ERR_HIP:
+            _check(r, "wr_prepare")
+        return r, dt.value, cell.value
+
+    def advance(self, completed, next_grid_time, dt_grid):
+        n = ctypes.c_int()
+        r = self._lib.wr_advance(self._h, completed, next_grid_time, dt_grid, ctypes.byref(n))
+        if r <= ERR_HIP:
+            _check(r, "wr_advance")
+        return r, n.value
+
+    # -- device building blocks (verification) ---------------------------------
+    def eval_rhs(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1, NEQS)
+        out = np.zeros_like(y)
+        _check(self._lib.wr_eval_rhs(self._h, _ptr(y), len(y), _ptr(out)), "wr_eval_rhs")
+        return out
+
+    def eval_jac(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1, NEQS)
+        out = np.zeros((len(y), NEQS, NEQS))
+        _check(self._lib.wr_eval_jac(self._h, _ptr(y), len(y), _ptr(out)), "wr_eval_jac")
+        return out
+
+    def eval_lu(self, a):
+        a = np.array(a, dtype=np.float64).reshape(-1, NEQS, NEQS).copy()
+        ipvt = np.zeros((len(a), NEQS), dtype=np.int32)
+        info = np.zeros(len(a), dtype=np.int32)
+        _check(self._lib.wr_eval_lu(self._h, _ptr(a), len(a), _ptr(ipvt), _ptr(info)),
+               "wr_eval_lu")
+        return a, ipvt, info
+
+    def eval_lu_solve(self, lu, ipvt, x):
+        lu = np.ascontiguousarray(lu, dtype=np.float64).reshape(-1, NEQS, NEQS)
+        ipvt = np.ascontiguousarray(ipvt, dtype=np.int32).reshape(-1, NEQS)
+        x = np.array(x, dtype=np.float64).reshape(-1, NEQS).copy()
+        _check(self._lib.wr_eval_lu_solve(self._h, _ptr(lu), _ptr(ipvt), len(x), _ptr(x)),
+               "wr_eval_lu_solve")
+        return x
+
+    def eval_integrate(self, y, tout, dt0, max_steps=0):
+        y = np.array(y, dtype=np.float64).reshape(-1, NEQS).copy()
+        n = len(y)
+        tout = np.ascontiguousarray(np.broadcast_to(tout, (n,)), dtype=np.float64)
+        dt0 = np.ascontiguousarray(np.broadcast_to(dt0, (n,)), dtype=np.float64)
+        counters = np.zeros((n, 7), dtype=np.int32)
+        t_end = np.zeros(n)
+        dt_end = np.zeros(n)
+        result = np.zeros(n, dtype=np.int32)
+        _check(self._lib.wr_eval_integrate(self._h, _ptr(y), _ptr(tout), _ptr(dt0), n, max_steps,
+                                           _ptr(counters), _ptr(t_end), _ptr(dt_end),
+                                           _ptr(result)), "wr_eval_integrate")
+        return {"y": y, "counters": counters, "t_end": t_end, "dt_end": dt_end, "result": result}
