@@ -1,0 +1,176 @@
+"""Benchmark: cell-advances/s of the 64^3 gridwide-independent collapse run.
+
+One bench step = one complete collapse run of the reference's default
+configuration -- 64^3 = 262,144 cells, make_collapse_state initial state,
+density perturbations every 20 steps, at most 1000 outer steps
+(prepare -> grid-wide minimum dt -> ROS2S advance), the run the reference
+times on H200 at 17.23 s (docs/performance/optimization-results.md:48-55).
+Cell states are uploaded to HBM before each timed run; the timed region is
+the run itself (device-resident outer loop, host polls a done flag).
+
+Prints one JSON line (see the keys below).  `value` = sum over outer steps of
+integrated cells / wall time of the K timed runs.  `roofline` is the FP64
+roofline of the advance kernel: algorithmic FP64 operations (static counts of
+the generated network and of the LU/solve/stage arithmetic, times the ROS2S
+counters of the run) per advance launch / its mean duration measured with HIP
+events on the launch stream.  `cpu_baseline` times the reference's own CPU
+build (oracle/_ref, single-threaded) on a bounded sample of the same
+workload.
+"""
+import argparse
+import importlib
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X vector FP64 (= FP64 matrix), dense
+
+
+def network_op_counts():
+    path = os.path.join(REPO, "warp-reactor_amd", "csrc", "_gen", "chem_counts.json")
+    with open(path) as f:
+        c = json.load(f)
+    flops = ("add", "sub", "mul", "div", "sqrt", "exp", "log")
+    return {k: sum(c[k].get(op, 0) for op in flops) for k in ("chem_fused", "chem_rhs")}
+
+
+def work_model():
+    """FP64 operations per unit of ROS2S work (transcendentals count 1)."""
+    n = 15
+    eos = 2 * 14 + 14 + 5 * 14 + 4 + 2  # density sum, abarinv/gammasinv sums, T = e/(...)
+    lu = sum(1 + (n - 1 - k) + 2 * (n - 1 - k) ** 2 for k in range(n - 1))
+    solve = 2 * sum(n - 1 - k for k in range(n - 1)) + n + 2 * sum(range(n))
+    stage = 476  # stage vectors, error norm, controller (per attempt)
+    c = network_op_counts()
+    return {"jac_eval": c["chem_fused"] + eos, "rhs_eval": c["chem_rhs"] + eos,
+            "decomp": n + lu, "solve": solve, "attempt": stage}
+
+
+def run_flops(stats_before, stats_after):
+    """Algorithmic FP64 operations of a run from the ROS2S counters:
+    per J evaluation the fused J + f(y); per attempt two more RHS, the
+    stage vectors and three solves; per decomposition E and its LU."""
+    w = work_model()
+    d = {f: int(stats_after[f].sum()) - int(stats_before[f].sum()) for f in stats_after.dtype.names}
+    attempts = d["internal_steps"]
+    return (d["jacobian_calls"] * w["jac_eval"] + 2 * attempts * w["rhs_eval"]
+            + d["decompositions"] * w["decomp"] + d["linear_solves"] * w["solve"]
+            + attempts * w["attempt"]), d
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference CPU binary (oracle/_ref/reproducer_cpu, single thread) on
+    a bounded sample: --grid 1 runs (438 gridwide-identical outer steps each)
+    repeated for about budget_s seconds."""
+    exe = os.path.join(REPO, "oracle", "_ref", "reproducer_cpu")
+    if not os.path.exists(exe):
+        return None
+    runs, advances, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        out = subprocess.run([exe, "--grid", "1", "--no-compare-final-state"], capture_output=True,
+                             text=True, cwd="/tmp", timeout=120)
+        if out.returncode != 0:
+            return None
+        steps = [int(l.split(":")[1]) for l in out.stdout.splitlines()
+                 if l.startswith("completed global collapse steps")]
+        advances += steps[0]
+        runs += 1
+    el = time.perf_counter() - t0
+    return {"value": advances / el, "unit": "cell-advances/s", "cores": 1, "kind": "reference",
+            "sample": f"{runs} runs of the reference CPU build at --grid 1 (1 cell x 438 outer "
+                      f"steps, legacy-local = gridwide for one cell), {el:.1f} s on 1 core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--max-outer", type=int, default=1000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    wr = importlib.import_module("warp-reactor_amd")
+    n = args.grid ** 3
+    init = wr.make_collapse_states(n)
+    ctx = wr.Context(n)
+    # prime: load the kernels with one outer step (not timed, not counted)
+    ctx.upload(init)
+    ctx.run(perturb=True, max_steps=1)
+    ctx.set_timing(True)
+
+    def one_run():
+        ctx.upload(init)  # inputs resident in HBM before the clock starts
+        t0 = time.perf_counter()
+        info = ctx.run(perturb=True, max_steps=args.max_outer)
+        return info, time.perf_counter() - t0
+
+    for _ in range(args.warmup):
+        one_run()
+    cells = np.zeros(n, dtype=wr.CELL_DTYPE)
+    total_adv, total_s, adv_ms, adv_launches, infos = 0, 0.0, 0.0, 0, []
+    for _ in range(args.steps):
+        info, el = one_run()
+        ms, nl = ctx.advance_timing()
+        adv_ms += ms
+        adv_launches += nl
+        total_adv += info["cell_advances"]
+        total_s += el
+        infos.append(info)
+        print(f"# run: {el:.3f} s, outer steps {info['completed_global_steps']}, "
+              f"cell-advances {info['cell_advances']}, status {info['status']}, "
+              f"advance kernel {ms:.1f} ms over {nl} launches", file=sys.stderr, flush=True)
+    ctx.download(cells)
+    flops, counters = run_flops(init["stats"], cells["stats"])
+    mean_launch_s = adv_ms / 1e3 / max(adv_launches, 1)
+    per_launch_flops = flops / max(infos[-1]["completed_global_steps"], 1)
+    achieved = per_launch_flops / mean_launch_s / 1e12 if adv_launches else None
+    value = total_adv / total_s
+    line = {
+        "metric": "cell-advances/s, 64^3 gridwide-independent ROS2S collapse (plain FP64)",
+        "value": value,
+        "unit": "cell-advances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total_s / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: reference initial state (make_collapse_state) with splitmix64 "
+                "density perturbations",
+        "config": {"workload": f"primordial collapse {args.grid}^3 = {n} cells, <= "
+                               f"{args.max_outer} outer steps, perturb every 20, plain FP64",
+                   "cells": n, "outer_steps": infos[-1]["completed_global_steps"],
+                   "cell_advances_per_run": infos[-1]["cell_advances"],
+                   "run_seconds": total_s / args.steps,
+                   "h200_reference_run_seconds": 17.2347},
+        "roofline": {"bound": "mfma", "compute_pipe": "VALU FP64 (MI355X FP64 peak 78.6 TFLOP/s, "
+                     "vector = matrix)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
+                     "traffic": None, "kernel": "k_advance",
+                     "mean_launch_ms": mean_launch_s * 1e3,
+                     "flops_per_launch": per_launch_flops},
+        "counters": counters,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -110,7 +110,7 @@ typedef struct wr_run_info {
     int32_t completed_global_steps;
     int32_t failed_step;        /* outer step of the failure, or -1 */
     int32_t failed_phase;       /* 1 = prepare, 2 = advance, 0 = none */
-    int32_t reserved_;
+    int32_t failed_cell;        /* cell whose failure ended the run, or -1 */
     double grid_time;           /* physical time of the grid after the run */
     uint64_t cell_advances;     /* sum over steps of integrated cells */
 } wr_run_info;
@@ -137,11 +137,19 @@ int wr_download_cells(wr_context *ctx, wr_collapse_state *cells, int num_cells);
 
 /* Run the outer loop on the resident cells from global step
  * `completed_global_steps` and grid time `grid_time` (normally 0, 0) for at
- * most `max_steps` steps (<= 0: 1000).  Host-side control is device resident:
+ * most `max_steps` steps (<= 0: 1000).  The outer-step index used by the
+ * perturbation schedule is the completed-step count, as in the reference
+ * loop (reproducer.cpp:7859, 7957).  Host-side control is device resident:
  * the host only polls a completion flag every few steps. */
 int wr_run(wr_context *ctx, int perturb, int max_steps, int completed_global_steps,
            double grid_time);
 int wr_run_info_get(const wr_context *ctx, wr_run_info *out);
+/* dt_grid of each outer step of the last wr_run (the host loop's dt_grid,
+ * reproducer.cpp:7888-7897), in step order: copies min(count, max_entries)
+ * values, *count = number of steps recorded (at most 4096).  With the initial
+ * states this determines every cell's trajectory independently of the others,
+ * so single cells can be replayed and checked. */
+int wr_dt_history(const wr_context *ctx, double *out, int max_entries, int *count);
 
 /* One kernel of the outer step each, for callers that drive the loop:
  * prepare (reproducer.cpp:7696) + device min reduction (7888-7896);
@@ -164,6 +172,9 @@ int wr_last_advance_timing(const wr_context *ctx, double *total_ms, int *launche
  * y: n x 15 row-major host arrays; outputs likewise.  Each runs the same
  * device functions the advance kernel uses. */
 int wr_eval_rhs(wr_context *ctx, const double *y, int n, double *dydt);
+/* Device libm on x[n]: op 0 exp, 1 log, 2 cbrt, 3 sqrt as the advance kernel
+ * evaluates them; 4 exp, 5 log, 6 cbrt of the ROCm device library. */
+int wr_eval_math(wr_context *ctx, int op, const double *x, int n, double *out);
 int wr_eval_jac(wr_context *ctx, const double *y, int n, double *jac /* n x 225 */);
 int wr_eval_lu(wr_context *ctx, double *a /* n x 225, in/out */, int n, int32_t *ipvt /* n x 15 */,
                int32_t *info /* n */);

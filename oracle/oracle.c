@@ -57,6 +57,20 @@ enum { OR_SUCCESS = 1, OR_BAD_INPUTS = -1, OR_DT_UNDERFLOW = -2, OR_TOO_MANY_STE
        OR_TOO_MUCH_ACCURACY = -5, OR_LU_ERROR = -7 };
 
 /* ---- generated network ------------------------------------------------- */
+/* libm: the host's (glibc, as the reference build) by default; with
+ * -DOR_WRMATH the deterministic exp/log/cbrt the GPU path uses
+ * (warp-reactor_amd/csrc/wr_math.h), so that this build and the GPU compute
+ * the same bits and differ from the default build only through libm. */
+#ifdef OR_WRMATH
+#include "../warp-reactor_amd/csrc/wr_math.h"
+#define CHEM_EXP(x) wr_exp(x)
+#define CHEM_LOG(x) wr_log(x)
+#define OR_CBRT(x) wr_cbrt(x)
+#else
+#define CHEM_EXP(x) exp(x)
+#define CHEM_LOG(x) log(x)
+#define OR_CBRT(x) cbrt(x)
+#endif
 #define CHEM_FN static inline
 #define CHEM_SINK_PARAMS , double *jsink
 #define CHEM_JAC_STORE(r, c, v) (jsink[(r) * NEQ + (c)] = (v))
@@ -362,7 +376,7 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                 }
                 err = sqrt(err / (double)NEQ);
 
-                const double raw_fac = cbrt(err) / k_safe;
+                const double raw_fac = OR_CBRT(err) / k_safe;
                 const double fac_step = ref_max(1.0 / k_fac_max, ref_min(1.0 / k_fac_min, raw_fac));
                 double hnew = h / fac_step;
                 if (err <= 1.0) {
@@ -372,7 +386,7 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                         const double facgus = ref_max(
                             1.0 / k_fac_max,
                             ref_min(1.0 / k_fac_min,
-                                    (hacc / h) * cbrt((err * err) / erracc) / k_safe));
+                                    (hacc / h) * OR_CBRT((err * err) / erracc) / k_safe));
                         hnew = h / ref_max(fac_step, facgus);
                     }
                     hacc = h;
@@ -598,3 +612,31 @@ int or_sizeof_cell(void) { return (int)sizeof(or_cell); }
 
 /* eos_rt on a burn_t laid out as rho, T, e, xn[14] (reproducer.cpp:306-309) */
 void or_eos_rt(double *burn) { eos_rt((or_burn_t *)burn); }
+
+/* The libm this build evaluates the network with (for the libm harness). */
+double or_libm_exp(double x) { return CHEM_EXP(x); }
+double or_libm_log(double x) { return CHEM_LOG(x); }
+double or_libm_cbrt(double x) { return OR_CBRT(x); }
+
+/* Replay cells of a gridwide run one by one: given the run's dt_grid of every
+ * outer step (the grid-wide minimum the host loop took, reproducer.cpp:7888-
+ * 7897), each cell's trajectory no longer depends on the other cells.  Cell k
+ * has grid index index[k]; status[k] receives its first failure code or
+ * OR_SUCCESS.  Steps start at completed0 with grid time time0. */
+void or_replay_cells(or_cell *cells, const int *index, int m, const double *dt_hist, int nsteps,
+                     int perturb, int completed0, double time0, int *status) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < m; ++k) {
+        double t = time0;
+        int f = OR_SUCCESS;
+        for (int s = 0; s < nsteps && f == OR_SUCCESS; ++s) {
+            const int step = completed0 + s;
+            or_prepare_cell(&cells[k], index[k], step, t, step, perturb, &f);
+            if (f != OR_SUCCESS) break;
+            const double next = t + dt_hist[s];
+            or_advance_cell(&cells[k], step, next, dt_hist[s], &f);
+            t = next;
+        }
+        status[k] = f;
+    }
+}

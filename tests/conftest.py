@@ -29,6 +29,12 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def oracle_wr():
+    import oracle_lib
+    return oracle_lib.OracleWrMath()
+
+
+@pytest.fixture(scope="session")
 def ref():
     import oracle_lib
     r = oracle_lib.Reference.load()

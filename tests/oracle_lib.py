@@ -8,6 +8,7 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+ORACLE_WR_SO = os.path.join(REPO, "oracle", "_build", "liboracle_wrmath.so")
 REF_SO = os.path.join(REPO, "oracle", "_ref", "libwr_ref.so")
 REF_CLI = os.path.join(REPO, "oracle", "_ref", "reproducer_cpu")
 
@@ -41,6 +42,9 @@ class _Cpu:
             "advance_cell": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
             "run_gridwide": (_I, [_VP, _I, _I, _I, ctypes.POINTER(_I)]),
         }
+        if p == "or_":
+            sig.update({"libm_exp": (_D, [_D]), "libm_log": (_D, [_D]), "libm_cbrt": (_D, [_D]),
+                        "replay_cells": (None, [_VP, _VP, _I, _VP, _I, _I, _I, _D, _VP])})
         for name, (res, args) in sig.items():
             fn = getattr(self.lib, p + name)
             fn.restype = res
@@ -96,6 +100,17 @@ class _Cpu:
         cells["reserved_"] = 0
         return cells
 
+    def replay_cells(self, cells, index, dt_hist, perturb=True, completed0=0, time0=0.0):
+        """Replay cells (their initial states in `cells`, grid indices `index`)
+        through the given dt_grid sequence; returns per-cell status codes."""
+        index = np.ascontiguousarray(index, dtype=np.int32)
+        dt_hist = np.ascontiguousarray(dt_hist, dtype=np.float64)
+        status = np.zeros(len(cells), dtype=np.int32)
+        self._replay_cells(_p(cells), _p(index), len(cells), _p(dt_hist), len(dt_hist),
+                           int(perturb), completed0, time0, _p(status))
+        cells["reserved_"] = 0
+        return status
+
     def burn_to_y(self, burn):
         """y = (xn, e) with e from eos_rt, as burn_ros2s builds it (7595-7606)."""
         b = np.array(burn, dtype=np.float64).copy()
@@ -136,6 +151,20 @@ class Oracle(_Cpu):
         if not os.path.exists(ORACLE_SO):
             raise RuntimeError(f"oracle not built: {ORACLE_SO} (make -C oracle)")
         super().__init__(ORACLE_SO)
+
+
+class OracleWrMath(_Cpu):
+    """The oracle evaluated with the GPU path's exp/log/cbrt (wr_math.h)."""
+    prefix = "or_"
+
+    def __init__(self):
+        if not os.path.exists(ORACLE_WR_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_WR_SO} (make -C oracle)")
+        super().__init__(ORACLE_WR_SO)
+
+    def libm(self, op, x):
+        fn = {"exp": self._libm_exp, "log": self._libm_log, "cbrt": self._libm_cbrt}[op]
+        return np.array([fn(float(v)) for v in np.ravel(x)])
 
 
 class Reference(_Cpu):

@@ -1,12 +1,14 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle.
 
-Tolerances.  Integer/index work (LU pivots, info codes, step counters,
-completed steps) must be identical.  FP64 values: the LU and the triangular
-solves involve no libm calls and must be bit-identical.  The network RHS/J
-use the device libm for exp/log (ROCm ocml), which may differ from glibc by
-an ulp, so single evaluations are held to a relative 1e-11 of the output's
-magnitude scale; integrated states to the north-star bar of 1e-9 relative per
-species above a 1e-20 abundance floor.
+The device evaluates exp/log/cbrt with wr_math.h (correctly rounded basic
+operations only).  The oracle built with the same functions
+(liboracle_wrmath.so) therefore has to be reproduced bit for bit: RHS, J,
+LU, solves, whole ROS2S integrations (all seven counters) and whole collapse
+runs.  The oracle built with the host libm is the reference's arithmetic
+(bit-identical to the reference CPU build, tests/test_oracle.py); against it
+the device differs only through libm, and the comparison uses the reference's
+own comparator (reproducer.cpp:8416-8549): species rtol 1e-3 (1.1e-3 H,
+3e-4 H2, 1e-4 for the others listed), atol 1e-4, T/e rtol 2e-4, rho 1e-4.
 """
 import numpy as np
 import pytest
@@ -14,14 +16,6 @@ import pytest
 from states import lu_test_matrices, random_matrices, random_states, trajectory_cases
 
 pytestmark = pytest.mark.gpu
-
-REL_EVAL = 1e-11
-REL_STATE = 1e-9
-FLOOR = 1e-20
-
-
-def _scaled_err(gpu, cpu, scale):
-    return np.abs(gpu - cpu) / np.maximum(scale, 1e-300)
 
 
 def bits(a):
@@ -31,33 +25,55 @@ def bits(a):
     return a.view(np.uint64)
 
 
-def test_device_rhs_matches_oracle(ctx, oracle):
-    ys = random_states(2048, oracle, seed=11)
+def assert_bits_equal(got, want, what):
+    g, w = bits(got), bits(want)
+    bad = np.nonzero(g != w)
+    assert len(bad[0]) == 0, f"{what}: {len(bad[0])} values differ, first at {tuple(b[0] for b in bad)}"
+
+
+# ---- libm ---------------------------------------------------------------------------
+
+def test_device_libm_matches_host_build(ctx, oracle_wr):
+    rng = np.random.default_rng(21)
+    xe = np.concatenate([rng.uniform(-745, 709, 20000), rng.uniform(-40, 40, 20000),
+                         [0.0, -0.0, 1e-300, 709.78, 709.8, -745.1, -746.0, np.inf, -np.inf, np.nan]])
+    xl = np.concatenate([10.0 ** rng.uniform(-320, 308, 20000), rng.uniform(0.5, 2.0, 20000),
+                         [5e-324, 1e-310, 1.0, 2.0, 0.0, -1.0, np.inf, np.nan]])
+    xc = np.concatenate([10.0 ** rng.uniform(-30, 30, 10000), -10.0 ** rng.uniform(-30, 30, 1000),
+                         [0.0, -0.0, 1e-310, 8.0, -27.0, np.inf, np.nan]])
+    for op, x in (("exp", xe), ("log", xl), ("cbrt", xc)):
+        assert_bits_equal(ctx.eval_math(op, x), oracle_wr.libm(op, x), op)
+    xs = 10.0 ** rng.uniform(-300, 300, 20000)
+    assert_bits_equal(ctx.eval_math("sqrt", xs), np.sqrt(xs), "sqrt (IEEE correctly rounded)")
+
+
+def test_ocml_versus_host_libm_report(ctx):
+    """Not a gate on the product: how far the ROCm device libm is from the
+    host libm, in ulps."""
+    rng = np.random.default_rng(22)
+    x = rng.uniform(-40, 40, 50000)
+    d = np.abs(ctx.eval_math("ocml_exp", x).view(np.int64) - np.exp(x).view(np.int64))
+    y = 10.0 ** rng.uniform(-30, 30, 50000)
+    dl = np.abs(ctx.eval_math("ocml_log", y).view(np.int64) - np.log(y).view(np.int64))
+    print(f"ocml exp: identical {np.mean(d == 0):.4f}, max ulp {d.max()}; "
+          f"ocml log: identical {np.mean(dl == 0):.4f}, max ulp {dl.max()}")
+    assert d.max() <= 4 and dl.max() <= 4
+
+
+# ---- network, LU, integrator: bit-exact against the oracle ------------------------------
+
+def test_device_rhs_bit_exact(ctx, oracle_wr):
+    ys = random_states(2048, oracle_wr, seed=11, t_range=(0.0, 5.5))
     got = ctx.eval_rhs(ys)
-    want = np.array([oracle.rhs(y) for y in ys])
-    same = np.mean(np.all(bits(got) == bits(want), axis=1))
-    # scale: magnitude of the largest contribution is not visible; use |f| and
-    # the row's largest |f_i| * 1e-6 as a cancellation allowance
-    scale = np.maximum(np.abs(want), 1e-6 * np.abs(want).max(axis=1, keepdims=True))
-    err = _scaled_err(got, want, scale)
-    print(f"rhs: bitwise-identical rows {same:.4f}, max scaled err {err.max():.3e}")
-    assert np.all(np.isfinite(got) == np.isfinite(want))
-    fin = np.isfinite(want)
-    assert err[fin].max() <= REL_EVAL
+    want = np.array([oracle_wr.rhs(y) for y in ys])
+    assert_bits_equal(got, want, "rhs")
 
 
-def test_device_jacobian_matches_oracle(ctx, oracle):
-    ys = random_states(1024, oracle, seed=12)
+def test_device_jacobian_bit_exact(ctx, oracle_wr):
+    ys = random_states(1024, oracle_wr, seed=12, t_range=(0.0, 5.5))
     got = ctx.eval_jac(ys)
-    want = np.array([oracle.jac(y) for y in ys])
-    same = np.mean(np.all(got.reshape(len(ys), -1).view(np.uint64) ==
-                          want.reshape(len(ys), -1).view(np.uint64), axis=1))
-    scale = np.maximum(np.abs(want), 1e-6 * np.abs(want).max(axis=(1, 2), keepdims=True))
-    err = _scaled_err(got, want, scale)
-    fin = np.isfinite(want)
-    print(f"jac: bitwise-identical matrices {same:.4f}, max scaled err {err[fin].max():.3e}")
-    assert np.all(np.isfinite(got) == fin)
-    assert err[fin].max() <= REL_EVAL
+    want = np.array([oracle_wr.jac(y) for y in ys])
+    assert_bits_equal(got, want, "jacobian")
 
 
 def test_device_lu_bit_exact(ctx, oracle):
@@ -87,71 +103,87 @@ def test_device_lu_solve_bit_exact(ctx, oracle):
 
 
 def test_lu_reference_cases_info(ctx):
-    """Known answers of the reference's LU cases: identity and forced swaps
-    are regular, the zero last row is singular at column 15."""
+    """The reference's LU cases (tests/gpu/test_warp_lu.mojo:19-47): identity,
+    zero last row (info 15), zero first row after the forced swap (info 2),
+    pivot tie broken by the first strict maximum, regular dominant cases."""
     mats, _ = lu_test_matrices()
     _, ip, info = ctx.eval_lu(mats)
     assert list(info) == [0, 15, 2, 0, 0, 0, 0, 0]
-    assert ip[2][0] == 1 and ip[3][0] == 1  # |-3| is the first strict maximum
+    assert ip[2][0] == 1 and ip[3][0] == 1
     assert list(ip[0]) == list(range(15))
 
 
-def test_device_integrate_matches_oracle(ctx, oracle, wr):
-    cases = trajectory_cases(oracle, wr.CELL_DTYPE, n_cells=6)
-    ys = np.array([oracle.burn_to_y(b) for b, _ in cases])
+def test_device_integrate_bit_exact(ctx, oracle_wr, wr):
+    cases = trajectory_cases(oracle_wr, wr.CELL_DTYPE, n_cells=6)
+    assert len(cases) >= 30
+    ys = np.array([oracle_wr.burn_to_y(b) for b, _ in cases])
     dts = np.array([dt for _, dt in cases])
     got = ctx.eval_integrate(ys, dts, dts)
-    n_same = 0
-    worst = 0.0
     for k in range(len(cases)):
-        want = oracle.integrate(ys[k], dts[k], dts[k])
+        want = oracle_wr.integrate(ys[k], dts[k], dts[k])
         assert got["result"][k] == want["result"] == 1
-        counters_same = np.array_equal(got["counters"][k], want["counters"])
-        n_same += counters_same
-        mask = np.abs(want["y"]) > FLOOR
-        rel = np.abs(got["y"][k] - want["y"])[mask] / np.abs(want["y"][mask])
-        worst = max(worst, rel.max())
-    print(f"integrate: {len(cases)} cases, identical counters {n_same}, worst rel {worst:.3e}")
-    assert n_same == len(cases)
-    assert worst <= REL_STATE
+        assert np.array_equal(got["counters"][k], want["counters"]), k
+        assert_bits_equal(got["y"][k], want["y"], f"integrate case {k}")
+        assert got["t_end"][k] == want["t_end"] and got["dt_end"][k] == want["dt_end"]
 
 
-def test_grid1_known_answer(wr, oracle):
-    """--grid 1: 438 global steps (DESIGN.md:855-858 of the reference) and the
-    oracle's final state and counters."""
+def test_device_integrate_fault_codes(ctx, oracle_wr, wr):
+    """TOO_MANY_STEPS forced from the host (max_steps = 3) and BAD_INPUTS for
+    tout <= t, with the counters the reference leaves."""
+    cases = trajectory_cases(oracle_wr, wr.CELL_DTYPE, n_cells=1, steps=(0, 20))
+    y = oracle_wr.burn_to_y(cases[0][0])
+    dt = cases[0][1]
+    got = ctx.eval_integrate(np.array([y, y]), np.array([dt, 0.0]), np.array([dt, dt]), max_steps=3)
+    want = oracle_wr.integrate(y, dt, dt, max_steps=3)
+    assert got["result"][0] == want["result"] == -4
+    assert np.array_equal(got["counters"][0], want["counters"])
+    assert got["result"][1] == -1
+
+
+# ---- whole runs -----------------------------------------------------------------------
+
+def _cells_equal(a, b):
+    mask = np.ones(216, bool)
+    mask[156:160] = False
+    return np.array_equal(a.view(np.uint8).reshape(len(a), 216)[:, mask],
+                          b.view(np.uint8).reshape(len(b), 216)[:, mask])
+
+
+def test_grid1_known_answer_bit_exact(wr, oracle_wr):
+    """--grid 1: 438 global steps (reference DESIGN.md:855-858); final state and
+    counters identical to the oracle with the same libm."""
     cells = wr.make_collapse_states(1)
     r, completed = wr.run_cells(cells, perturb=True)
-    assert r == 1
-    assert completed == 438
-    ocells = oracle.make_cells(1, wr.CELL_DTYPE)
-    orr, ocompleted = oracle.run_gridwide(ocells, True)
+    assert r == 1 and completed == 438
+    ocells = oracle_wr.make_cells(1, wr.CELL_DTYPE)
+    orr, ocompleted = oracle_wr.run_gridwide(ocells, True)
     assert ocompleted == 438
-    g, o = cells[0], ocells[0]
-    assert g["completed_steps"] == o["completed_steps"] == 438
-    for f in wr.STATS_FIELDS:
-        assert g["stats"][f] == o["stats"][f], f
-    mask = o["current"]["xn"] > FLOOR
-    rel = np.abs(g["current"]["xn"] - o["current"]["xn"])[mask] / o["current"]["xn"][mask]
-    print(f"grid1: T gpu {g['current']['T']!r} oracle {o['current']['T']!r}; max species rel {rel.max():.3e}")
-    assert rel.max() <= REL_STATE
-    assert abs(g["current"]["T"] - o["current"]["T"]) <= REL_STATE * o["current"]["T"]
+    assert _cells_equal(cells, ocells)
 
 
 @pytest.mark.parametrize("grid,perturb", [(2, True), (4, True), (3, False)])
-def test_small_grid_matches_oracle(wr, oracle, grid, perturb):
+def test_small_grid_bit_exact(wr, oracle_wr, grid, perturb):
     n = grid ** 3
     cells = wr.make_collapse_states(n)
     r, completed = wr.run_cells(cells, perturb=perturb)
-    ocells = oracle.make_cells(n, wr.CELL_DTYPE)
-    orr, ocompleted = oracle.run_gridwide(ocells, perturb)
+    ocells = oracle_wr.make_cells(n, wr.CELL_DTYPE)
+    orr, ocompleted = oracle_wr.run_gridwide(ocells, perturb)
     assert r == orr == 1
     assert completed == ocompleted
-    assert np.array_equal(cells["completed_steps"], ocells["completed_steps"])
-    stats_same = np.mean([all(cells[c]["stats"][f] == ocells[c]["stats"][f] for f in wr.STATS_FIELDS)
-                          for c in range(n)])
-    xg, xo = cells["current"]["xn"], ocells["current"]["xn"]
-    mask = xo > FLOOR
-    rel = np.abs(xg - xo)[mask] / xo[mask]
-    print(f"grid {grid}: identical counters in {stats_same:.3f} of cells, max rel {rel.max():.3e}")
-    assert stats_same == 1.0
-    assert rel.max() <= REL_STATE
+    assert _cells_equal(cells, ocells)
+
+
+def test_grid_against_reference_comparator(wr, oracle, tmp_path):
+    """Against the reference arithmetic (host libm), a 4^3 run passes the
+    reference's own final-state comparator."""
+    n, g = 64, 4
+    ocells = oracle.make_cells(n, wr.CELL_DTYPE)
+    oracle.run_gridwide(ocells, True)
+    path = str(tmp_path / "final_states_grid4_cpu.bin")
+    wr.write_final_states(ocells, g, path)
+    cells = wr.make_collapse_states(n)
+    r, completed = wr.run_cells(cells, perturb=True)
+    assert r == 1
+    summary = wr.compare_final_states(cells, g, path)
+    print("comparator vs host-libm oracle:", summary)
+    assert summary["pass"], summary["first_failure"]

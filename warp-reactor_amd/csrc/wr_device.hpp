@@ -16,8 +16,23 @@
 #include <stdint.h>
 
 #include "wr_eos.hpp"
+#include "wr_math.h"
 
 namespace wr {
+
+// ---- libm ---------------------------------------------------------------------
+// exp/log/cbrt from wr_math.h: only correctly rounded operations, so a host
+// build of the same algorithm reproduces the device bits.  Building with
+// -DWR_DEVICE_LIBM_OCML selects the ROCm device library instead.
+#if defined(WR_DEVICE_LIBM_OCML)
+#define CHEM_EXP(x) exp(x)
+#define CHEM_LOG(x) log(x)
+#define WR_CBRT(x) cbrt(x)
+#else
+#define CHEM_EXP(x) wr_exp(x)
+#define CHEM_LOG(x) wr_log(x)
+#define WR_CBRT(x) wr_cbrt(x)
+#endif
 
 // ---- generated network --------------------------------------------------------
 #define CHEM_FN __device__ __forceinline__
@@ -280,7 +295,7 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
             c.n_step = n_step;
             err = sqrt(err / static_cast<double>(kNeqs));
 
-            const double raw_fac = cbrt(err) / kSafe;
+            const double raw_fac = WR_CBRT(err) / kSafe;
             const double fac_step = rmax(1.0 / kFacMax, rmin(1.0 / kFacMin, raw_fac));
             double hnew = h / fac_step;
             if (err <= 1.0) {
@@ -289,7 +304,7 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
                 if (n_accept > 1) {
                     const double facgus =
                         rmax(1.0 / kFacMax,
-                             rmin(1.0 / kFacMin, (hacc / h) * cbrt((err * err) / erracc) / kSafe));
+                             rmin(1.0 / kFacMin, (hacc / h) * WR_CBRT((err * err) / erracc) / kSafe));
                     hnew = h / rmax(fac_step, facgus);
                 }
                 hacc = h;

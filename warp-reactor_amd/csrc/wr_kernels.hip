@@ -59,22 +59,26 @@ struct Ctl {
     int integrated;
     int completed;
     int step;
-    int max_steps;
+    int step_limit;  // the run stops once step reaches this
     int done;
     int perturb;
     int stop_reason;
     int failed_step;
     int failed_phase;
     int limiting_cell;
-    int pad_;
+    int failed_cell;
+    int step0;          // step index at the start of the run
+    int hist_cap;       // capacity of dt_hist
+    double* dt_hist;    // dt_grid of every outer step of the run (device)
 };
 
 constexpr int kPrepBlock = 256;
 constexpr int kAdvBlock = 64;
 constexpr int kPlanBlock = 1024;
 
-__device__ __forceinline__ void set_failure(Ctl* ctl, int code) {
-    atomicCAS(&ctl->failure, kSuccess, code);
+// First failure wins (reproducer.cpp:7716, 7724, 7782); the winner records its cell.
+__device__ __forceinline__ void set_failure(Ctl* ctl, int code, int cell) {
+    if (atomicCAS(&ctl->failure, kSuccess, code) == kSuccess) ctl->failed_cell = cell;
 }
 
 // (dt, cell) pair order: smaller dt, then smaller cell.
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(kPrepBlock) void k_prepare(DevCells s, Ctl* ctl, do
         const int done_steps = s.completed[c];
         if (done_steps >= completed) {
             if (done_steps != completed || s.time[c] != ctl->grid_time) {
-                set_failure(ctl, kBadInputs);
+                set_failure(ctl, kBadInputs, c);
             } else {
                 double xn[kNumSpec];
 #pragma unroll
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(kPrepBlock) void k_prepare(DevCells s, Ctl* ctl, do
                 }
                 const double dt = collapse_timestep(xn);
                 if (!valid_positive(dt) || !valid_positive(driver)) {
-                    set_failure(ctl, kBadInputs);
+                    set_failure(ctl, kBadInputs, c);
                 } else if (dt < 10.0) {
                     const double tff = dt / 1.0e-1;
                     driver += dt * (driver / tff);
@@ -246,6 +250,18 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(Ctl* ctl, const double* par
     ctl->next_grid_time = next;
     ctl->limiting_cell = cell;
     ctl->integrated = 0;
+    const int k = ctl->step - ctl->step0;
+    if (ctl->dt_hist && k >= 0 && k < ctl->hist_cap) ctl->dt_hist[k] = dt;
+}
+
+// The integrator as a real (non-inlined) function: its register allocation
+// is separate from the callers', which keeps the kernels' own control flow
+// and state out of the integrator's spill code.
+__device__ __noinline__ int integrate_cell(double (&y)[kNeqs], double tout, double dt0,
+                                           int max_steps, Counters& c, double* __restrict__ J,
+                                           double* __restrict__ LU, int stride, double& t_end,
+                                           double& dt_end) {
+    return ros2s_integrate(y, tout, dt0, max_steps, c, J, LU, stride, t_end, dt_end);
 }
 
 // advance_collapse_gridwide_kernel (reproducer.cpp:7736-7785), one thread per cell.
@@ -281,7 +297,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance(DevCells s, DevWs ws, Ctl
     y[kNumSpec] = e0;
     Counters cnt;
     double t_end, dt_end;
-    const int result = ros2s_integrate(y, dt_grid, dt_grid, 10000000, cnt, ws.J + c, ws.LU + c,
+    const int result = integrate_cell(y, dt_grid, dt_grid, 10000000, cnt, ws.J + c, ws.LU + c,
                                        ws.cap, t_end, dt_end);
     const int cv[7] = {cnt.n_step, cnt.n_rhs, cnt.n_jac, cnt.n_decomp, cnt.n_solve,
                        cnt.n_accept, cnt.n_reject};
@@ -294,7 +310,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance(DevCells s, DevWs ws, Ctl
         s.e[c] = e0;
 #pragma unroll
         for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
-        set_failure(ctl, result);
+        set_failure(ctl, result, c);
         return;
     }
 #pragma unroll
@@ -328,7 +344,7 @@ __global__ void k_commit(Ctl* ctl) {
     ctl->grid_time = ctl->next_grid_time;
     ctl->completed += 1;
     ctl->step += 1;
-    if (ctl->step >= ctl->max_steps) {
+    if (ctl->step >= ctl->step_limit) {
         ctl->done = 1;
         ctl->stop_reason = WR_STOP_STEP_LIMIT;
     }
@@ -388,6 +404,25 @@ __global__ void k_eval_solve(const double* __restrict__ lu, const int* __restric
     for (int k = 0; k < kNeqs; ++k) x[i * kNeqs + k] = xx[k];
 }
 
+// Device libm on a batch: 0 exp, 1 log, 2 cbrt, 3 sqrt as the network uses
+// them (wr_math.h / IEEE sqrt), 4..6 the ROCm device library's exp, log, cbrt.
+__global__ void k_eval_math(int op, const double* __restrict__ x, int n, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    double r;
+    switch (op) {
+        case 0: r = wr_exp(v); break;
+        case 1: r = wr_log(v); break;
+        case 2: r = wr_cbrt(v); break;
+        case 3: r = sqrt(v); break;
+        case 4: r = exp(v); break;
+        case 5: r = log(v); break;
+        default: r = cbrt(v); break;
+    }
+    out[i] = r;
+}
+
 __global__ __launch_bounds__(kAdvBlock) void k_eval_integrate(double* __restrict__ y,
                                                               const double* __restrict__ tout,
                                                               const double* __restrict__ dt0, int n,
@@ -403,7 +438,7 @@ __global__ __launch_bounds__(kAdvBlock) void k_eval_integrate(double* __restrict
     for (int k = 0; k < kNeqs; ++k) yy[k] = y[i * kNeqs + k];
     Counters c;
     double te, de;
-    const int r = ros2s_integrate(yy, tout[i], dt0[i], max_steps, c, ws.J + i, ws.LU + i, ws.cap,
+    const int r = integrate_cell(yy, tout[i], dt0[i], max_steps, c, ws.J + i, ws.LU + i, ws.cap,
                                   te, de);
 #pragma unroll
     for (int k = 0; k < kNeqs; ++k) y[i * kNeqs + k] = yy[k];
@@ -457,6 +492,9 @@ struct wr_context {
     DevWs ws{};
     Ctl* ctl = nullptr;         // device
     Ctl* ctl_host = nullptr;    // pinned mirror
+    static constexpr int kHistCap = 4096;
+    double* dt_hist = nullptr;  // dt_grid per outer step of the last run
+    int hist_len = 0;
     double* part_dt = nullptr;  // per-prepare-block partials
     int* part_cell = nullptr;
     wr_collapse_state* staging = nullptr;  // device AoS staging buffer
@@ -486,6 +524,7 @@ void wr_context_destroy(wr_context* ctx) {
     (void)hipFree(ctx->ctl);
     (void)hipFree(ctx->part_dt);
     (void)hipFree(ctx->part_cell);
+    (void)hipFree(ctx->dt_hist);
     (void)hipFree(ctx->staging);
     if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
     for (int k = 0; k < wr_context::kChunk; ++k) {
@@ -541,6 +580,7 @@ int wr_context_create(int device, int capacity_cells, wr_context** out) {
         !hip_ok(hipHostMalloc(&ctx->ctl_host, sizeof(Ctl)), "hipHostMalloc(ctl)") ||
         !hip_ok(hipMalloc(&ctx->part_dt, nparts * sizeof(double)), "hipMalloc(partials)") ||
         !hip_ok(hipMalloc(&ctx->part_cell, nparts * sizeof(int)), "hipMalloc(partials)") ||
+        !hip_ok(hipMalloc(&ctx->dt_hist, wr_context::kHistCap * sizeof(double)), "hipMalloc(dt history)") ||
         !hip_ok(hipMalloc(&ctx->staging, cap * sizeof(wr_collapse_state)), "hipMalloc(staging)"))
         return cleanup(WR_ERR_HIP);
     for (int k = 0; k < wr_context::kChunk; ++k) {
@@ -591,8 +631,12 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.grid_time = grid_time;
     h.failure = kSuccess;
     h.completed = completed;
-    h.step = 0;
-    h.max_steps = max_steps;
+    h.step = completed;  // the reference's loop index equals the completed count
+    h.step_limit = completed + max_steps;
+    h.failed_cell = -1;
+    h.step0 = completed;
+    h.hist_cap = wr_context::kHistCap;
+    h.dt_hist = ctx->dt_hist;
     h.done = (max_steps <= 0) ? 1 : 0;
     h.perturb = perturb ? 1 : 0;
     h.stop_reason = (max_steps <= 0) ? WR_STOP_STEP_LIMIT : WR_STOP_NONE;
@@ -638,8 +682,16 @@ static void fill_info(wr_context* ctx) {
     i.completed_global_steps = h.completed;
     i.failed_step = h.failed_step;
     i.failed_phase = h.failed_phase;
+    i.failed_cell = h.failed_cell;
     i.grid_time = h.grid_time;
     i.cell_advances = h.cell_advances;
+    // steps whose dt_grid was planned: completed steps, plus one if the run
+    // ended in an advance phase
+    int len = h.step - h.step0;
+    if (h.stop_reason == WR_STOP_NONE_INTEGRATED ||
+        (h.stop_reason == WR_STOP_FAILURE && h.failed_phase == 2))
+        len += 1;
+    ctx->hist_len = len < 0 ? 0 : (len > wr_context::kHistCap ? wr_context::kHistCap : len);
 }
 
 int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_steps,
@@ -691,6 +743,19 @@ int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_ste
     }
     fill_info(ctx);
     return ctx->ctl_host->failure;
+}
+
+int wr_dt_history(const wr_context* ctx, double* out, int max_entries, int* count) {
+    if (!ctx || !count || (max_entries > 0 && !out))
+        return fail(WR_ERR_ARGUMENT, "wr_dt_history: bad argument");
+    const int n = ctx->hist_len < max_entries ? ctx->hist_len : max_entries;
+    *count = ctx->hist_len;
+    if (n > 0) {
+        WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+        WR_HIP(hipMemcpy(out, ctx->dt_hist, n * sizeof(double), hipMemcpyDeviceToHost),
+               "hipMemcpy(dt history)");
+    }
+    return WR_SUCCESS;
 }
 
 int wr_set_timing(wr_context* ctx, int enable) {
@@ -853,6 +918,23 @@ int wr_eval_jac(wr_context* ctx, const double* y, int n, double* jac) {
     WR_HIP(hipGetLastError(), "k_eval_jac launch");
     WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     WR_HIP(hipMemcpy(jac, dj.p, b * kNeqs, hipMemcpyDeviceToHost), "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+int wr_eval_math(wr_context* ctx, int op, const double* x, int n, double* out) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    if (op < 0 || op > 6) return fail(WR_ERR_ARGUMENT, "wr_eval_math: op must be 0..6");
+    DevBuf dx, dout;
+    const size_t b = static_cast<size_t>(n) * sizeof(double);
+    WR_HIP(hipMalloc(&dx.p, b), "hipMalloc");
+    WR_HIP(hipMalloc(&dout.p, b), "hipMalloc");
+    WR_HIP(hipMemcpy(dx.p, x, b, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_math, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, op,
+                       static_cast<double*>(dx.p), n, static_cast<double*>(dout.p));
+    WR_HIP(hipGetLastError(), "k_eval_math launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(out, dout.p, b, hipMemcpyDeviceToHost), "hipMemcpy");
     return WR_SUCCESS;
 }
 

@@ -89,8 +89,10 @@ class _Emitter:
             return f"-{a[0]}"
         if op == "abs":
             return f"fabs({a[0]})"
-        if op in ("exp", "log", "sqrt"):
-            return f"{op}({a[0]})"
+        if op in ("exp", "log"):
+            return f"CHEM_{op.upper()}({a[0]})"
+        if op == "sqrt":
+            return f"sqrt({a[0]})"
         binop = {"add": "+", "sub": "-", "mul": "*", "div": "/", "lt": "<", "le": "<=",
                  "gt": ">", "ge": ">=", "eq": "==", "ne": "!=", "and": "&&", "or": "||"}
         if op in binop:
@@ -156,7 +158,11 @@ def emit_header(ir_path, out_path, counts_path=None):
         f" * (network source sha256 {sha}). Do not edit.",
         " * Straight-line FP64 right-hand side and Jacobian of the 14-species +",
         " * energy primordial network (reproducer.cpp:341-6994). Compile with",
-        " * -ffp-contract=off. The includer defines CHEM_FN and CHEM_JAC_STORE. */",
+        " * -ffp-contract=off. The includer defines CHEM_FN, CHEM_JAC_STORE,",
+        " * CHEM_SINK_PARAMS and the exp/log it wants as CHEM_EXP / CHEM_LOG. */",
+        "#ifndef CHEM_EXP",
+        "#error define CHEM_EXP(x) and CHEM_LOG(x) before including the generated network",
+        "#endif",
         "#ifndef CHEM_FN",
         "#error define CHEM_FN before including the generated network",
         "#endif",

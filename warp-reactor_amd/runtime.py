@@ -43,7 +43,7 @@ class WarpReactorError(RuntimeError):
 class RunInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("stop_reason", ctypes.c_int32),
                 ("completed_global_steps", ctypes.c_int32), ("failed_step", ctypes.c_int32),
-                ("failed_phase", ctypes.c_int32), ("reserved_", ctypes.c_int32),
+                ("failed_phase", ctypes.c_int32), ("failed_cell", ctypes.c_int32),
                 ("grid_time", ctypes.c_double), ("cell_advances", ctypes.c_uint64)]
 
 
@@ -77,9 +77,11 @@ _SIGS = {
     "wr_run_info_get": (_I, [_VP, ctypes.POINTER(RunInfo)]),
     "wr_prepare": (_I, [_VP, _I, _D, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
+    "wr_dt_history": (_I, [_VP, _VP, _I, ctypes.POINTER(_I)]),
     "wr_set_timing": (_I, [_VP, _I]),
     "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_eval_rhs": (_I, [_VP, _VP, _I, _VP]),
+    "wr_eval_math": (_I, [_VP, _I, _VP, _I, _VP]),
     "wr_eval_jac": (_I, [_VP, _VP, _I, _VP]),
     "wr_eval_lu": (_I, [_VP, _VP, _I, _VP, _VP]),
     "wr_eval_lu_solve": (_I, [_VP, _VP, _VP, _I, _VP]),
@@ -210,6 +212,13 @@ class Context:
         _check(self._lib.wr_run_info_get(self._h, ctypes.byref(info)), "wr_run_info_get")
         return {f: getattr(info, f) for f, _ in RunInfo._fields_ if f != "reserved_"}
 
+    def dt_history(self):
+        out = np.zeros(4096)
+        n = ctypes.c_int()
+        _check(self._lib.wr_dt_history(self._h, _ptr(out), len(out), ctypes.byref(n)),
+               "wr_dt_history")
+        return out[:n.value].copy()
+
     def set_timing(self, enable=True):
         _check(self._lib.wr_set_timing(self._h, int(bool(enable))), "wr_set_timing")
 
@@ -239,6 +248,16 @@ class Context:
         y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1, NEQS)
         out = np.zeros_like(y)
         _check(self._lib.wr_eval_rhs(self._h, _ptr(y), len(y), _ptr(out)), "wr_eval_rhs")
+        return out
+
+    MATH_OPS = {"exp": 0, "log": 1, "cbrt": 2, "sqrt": 3, "ocml_exp": 4, "ocml_log": 5,
+                "ocml_cbrt": 6}
+
+    def eval_math(self, op, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        out = np.zeros_like(x)
+        _check(self._lib.wr_eval_math(self._h, self.MATH_OPS[op], _ptr(x), len(x), _ptr(out)),
+               "wr_eval_math")
         return out
 
     def eval_jac(self, y):
