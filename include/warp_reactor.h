@@ -161,6 +161,11 @@ int wr_prepare(wr_context *ctx, int completed_global_steps, double grid_time, in
 int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_time,
                double dt_grid, int *integrated);
 
+/* Advance kernel of subsequent steps: 1 (default) = queue-fed persistent
+ * kernel (lanes take the next cell when theirs finishes; register-budgeted
+ * network code); 0 = one thread per cell.  Results are identical. */
+int wr_set_advance_kernel(wr_context *ctx, int kernel);
+
 /* Time the advance kernel launches of subsequent wr_run calls with HIP events
  * recorded on the context's stream (off by default). */
 int wr_set_timing(wr_context *ctx, int enable);

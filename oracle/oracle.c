@@ -74,6 +74,13 @@ enum { OR_SUCCESS = 1, OR_BAD_INPUTS = -1, OR_DT_UNDERFLOW = -2, OR_TOO_MANY_STE
 #define CHEM_FN static inline
 #define CHEM_SINK_PARAMS , double *jsink
 #define CHEM_JAC_STORE(r, c, v) (jsink[(r) * NEQ + (c)] = (v))
+/* staged (register-budgeted) variants: a local staging array, outputs to jac/f */
+#define CHEM_STAGE_PARAMS , double *stg, double *jout, double *fout
+#define STAGE_STORE(s, v) (stg[(s)] = (v))
+#define STAGE_LOAD(s) (stg[(s)])
+#define CHEM_OUT(kind, ...) CHEM_OUT_##kind(__VA_ARGS__)
+#define CHEM_OUT_J(r, c, v) (jout[(r) * NEQ + (c)] = (v))
+#define CHEM_OUT_F(i, v) (fout[(i)] = (v))
 #include "_build/chem_network.h"
 
 /* ---- network constants and EOS (reproducer.cpp:199-339) ----------------- */
@@ -639,4 +646,45 @@ void or_replay_cells(or_cell *cells, const int *index, int m, const double *dt_h
         }
         status[k] = f;
     }
+}
+
+/* As or_replay_cells, also recording the ROS2S attempts each cell makes in
+ * each step: attempts[k * nsteps + s]. */
+void or_replay_cells_profile(or_cell *cells, const int *index, int m, const double *dt_hist,
+                             int nsteps, int perturb, int completed0, double time0, int *status,
+                             int *attempts) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < m; ++k) {
+        double t = time0;
+        int f = OR_SUCCESS;
+        for (int s = 0; s < nsteps; ++s) attempts[(size_t)k * nsteps + s] = 0;
+        for (int s = 0; s < nsteps && f == OR_SUCCESS; ++s) {
+            const int step = completed0 + s;
+            or_prepare_cell(&cells[k], index[k], step, t, step, perturb, &f);
+            if (f != OR_SUCCESS) break;
+            const double next = t + dt_hist[s];
+            const uint64_t before = cells[k].stats.internal_steps;
+            or_advance_cell(&cells[k], step, next, dt_hist[s], &f);
+            attempts[(size_t)k * nsteps + s] = (int)(cells[k].stats.internal_steps - before);
+            t = next;
+        }
+        status[k] = f;
+    }
+}
+
+/* The register-budgeted (staged) network functions the GPU kernels compile,
+ * evaluated here with a local staging array: they must reproduce chem_fused /
+ * chem_rhs bit for bit (only the evaluation order differs). */
+void or_jac_rhs_staged(const double *y, double *jac, double *dydt) {
+    double X[NSPEC];
+    const double T = temperature_from_y(y, X);
+    double stg[CHEM_FUSED_NSLOT];
+    chem_fused_staged(T, X, stg, jac, dydt);
+}
+
+void or_rhs_staged(const double *y, double *dydt) {
+    double X[NSPEC];
+    const double T = temperature_from_y(y, X);
+    double stg[CHEM_RHS_NSLOT];
+    chem_rhs_staged(T, X, stg, (double *)0, dydt);
 }

@@ -173,6 +173,43 @@ def test_small_grid_bit_exact(wr, oracle_wr, grid, perturb):
     assert _cells_equal(cells, ocells)
 
 
+def _run_context(wr, n, kernel, perturb=True, max_steps=1000):
+    cells = wr.make_collapse_states(n)
+    with wr.Context(n) as ctx:
+        ctx.set_advance_kernel(kernel)
+        ctx.upload(cells)
+        info = ctx.run(perturb=perturb, max_steps=max_steps)
+        ctx.download(cells)
+        dts = ctx.dt_history()
+    return cells, info, dts
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("grid,perturb", [(2, True), (3, False)])
+def test_advance_kernels_bit_exact(wr, oracle_wr, kernel, grid, perturb):
+    """Both advance kernels (one thread per cell; queue-fed persistent) give
+    the oracle's final states and counters bit for bit."""
+    n = grid ** 3
+    cells, info, dts = _run_context(wr, n, kernel, perturb)
+    ocells = oracle_wr.make_cells(n, wr.CELL_DTYPE)
+    orr, ocompleted = oracle_wr.run_gridwide(ocells, perturb)
+    assert info["status"] == orr == 1
+    assert info["completed_global_steps"] == ocompleted == len(dts)
+    assert _cells_equal(cells, ocells)
+
+
+def test_advance_kernels_agree_16cube(wr):
+    """16^3 cells, 300 outer steps: the queue-fed kernel (lanes refilled from
+    an atomic queue, cells finishing in any order) and the thread-per-cell
+    kernel produce identical grids, dt histories and run info."""
+    n = 16 ** 3
+    a, ia, da = _run_context(wr, n, 0, max_steps=300)
+    b, ib, db = _run_context(wr, n, 1, max_steps=300)
+    assert ia == ib
+    assert np.array_equal(da, db)
+    assert _cells_equal(a, b)
+
+
 def test_grid_against_reference_comparator(wr, oracle, tmp_path):
     """Against the reference arithmetic (host libm), a 4^3 run passes the
     reference's own final-state comparator."""

@@ -38,6 +38,16 @@ namespace wr {
 #define CHEM_FN __device__ __forceinline__
 #define CHEM_SINK_PARAMS , double* __restrict__ jsink, int jstride
 #define CHEM_JAC_STORE(r, c, v) (jsink[((r) * kNeqs + (c)) * jstride] = (v))
+// register-budgeted variants: staging slots and outputs in a per-thread SoA
+// workspace with the given stride
+#define CHEM_STAGE_PARAMS                                                                   \
+    , double* __restrict__ stg, int sstride, double* __restrict__ jout,                    \
+        double* __restrict__ fout
+#define STAGE_STORE(s, v) (stg[(s) * sstride] = (v))
+#define STAGE_LOAD(s) (stg[(s) * sstride])
+#define CHEM_OUT(kind, ...) CHEM_OUT_##kind(__VA_ARGS__)
+#define CHEM_OUT_J(r, c, v) (jout[((r) * kNeqs + (c)) * sstride] = (v))
+#define CHEM_OUT_F(i, v) (fout[(i) * sstride] = (v))
 #include "_gen/chem_network.h"
 #undef CHEM_JAC_STORE
 #undef CHEM_SINK_PARAMS
@@ -323,6 +333,177 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
             h = hnew;
             if (n_accept >= 1) c.n_reject += 1;
         }
+    }
+}
+
+// =====================================================================================
+// Queue-fed integrator: per-thread workspace, staged network, left-looking LU
+// =====================================================================================
+
+// Per-thread workspace, structure of arrays over the threads of a launch:
+// element e of thread t at base[e * stride + t] (a wave touches 512
+// contiguous bytes per element).
+struct ThreadWs {
+    double* J;      // [225]  Jacobian at y (kept for retries)
+    double* LU;     // [225]  factors of E = fac I - J
+    double* vec;    // [kVecLen] y, f0, ak1, ak2, ak3, ynew, dy
+    double* stage;  // [kStageSlots] staging slots of the network code
+    int stride;
+};
+constexpr int kVecY = 0, kVecF0 = 15, kVecAk1 = 30, kVecAk2 = 45, kVecAk3 = 60, kVecYnew = 75,
+              kVecDy = 90, kVecLen = 105;
+constexpr int kStageSlots = (CHEM_FUSED_NSLOT > CHEM_RHS_NSLOT) ? CHEM_FUSED_NSLOT : CHEM_RHS_NSLOT;
+
+#define WR_AT(base, e, stride) ((base)[(e) * (stride)])
+
+// J(y) and f(y) with the register-budgeted network code; y is read from and
+// J, f0 written to the workspace.  Not inlined: inside a persistent loop the
+// compiler would hoist the network's ~1100 FP64 constants out of the loop and
+// spill them (over 1700 SGPR spills).
+__device__ __noinline__ void ws_jac_rhs(const ThreadWs& w) {
+    double X[kNumSpec];
+    double yy[kNeqs];
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) yy[i] = WR_AT(w.vec, kVecY + i, w.stride);
+    const double T = state_from_y(yy, X);
+    double* __restrict__ stg = w.stage;
+    double* __restrict__ J = w.J;
+    double* __restrict__ F = w.vec + kVecF0 * w.stride;
+    const int stride = w.stride;
+    chem_fused_staged(T, X, stg, stride, J, F);
+}
+
+// f(ynew) into dy.
+__device__ __noinline__ void ws_rhs(const ThreadWs& w) {
+    double X[kNumSpec];
+    double yy[kNeqs];
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) yy[i] = WR_AT(w.vec, kVecYnew + i, w.stride);
+    const double T = state_from_y(yy, X);
+    double* __restrict__ stg = w.stage;
+    double* __restrict__ F = w.vec + kVecDy * w.stride;
+    const int stride = w.stride;
+    chem_rhs_staged(T, X, stg, stride, nullptr, F);
+}
+
+__device__ __forceinline__ int piv_get(uint64_t piv, int k) { return (int)((piv >> (4 * k)) & 15u); }
+
+// Row exchange k <-> p of a register vector (p >= k, dynamic), with selects;
+// skipped when no lane of the wave exchanges.
+__device__ __forceinline__ void swap_rows(double (&v)[kNeqs], int k, int p) {
+    if (__ballot(p != k) == 0) return;
+    const double vk = v[k];
+    double vp = vk;
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i)
+        if (i > k) vp = (p == i) ? v[i] : vp;
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i)
+        if (i > k) v[i] = (p == i) ? vk : v[i];
+    v[k] = vp;
+}
+
+// LU of E = fac I - J, left-looking (column by column), with exactly the
+// operations of the reference's right-looking LINPACK elimination
+// (reproducer.cpp:97-151): each element receives the same updates
+// a_ij += a_kj * m_ik in the same k order, pivots are the first strict
+// maximum, rows are exchanged from column k on and multipliers are stored
+// negated; a zero pivot skips its exchange, scaling and updates.  Column j
+// lives in registers; the multipliers of the first kCache columns stay in
+// registers too, the rest are re-read from LU.  Returns info; pivots packed
+// 4 bits per column in *piv.
+constexpr int kCache = 5;
+
+__device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& piv) {
+    const int st = w.stride;
+    const double* __restrict__ J = w.J;
+    double* __restrict__ LU = w.LU;
+    int info = 0;
+    uint32_t sing = 0;
+    piv = 0;
+    double mc[kCache][kNeqs];
+#pragma unroll
+    for (int j = 0; j < kNeqs; ++j) {
+        double col[kNeqs];
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) {
+            const double v = -WR_AT(J, i * kNeqs + j, st);
+            col[i] = (i == j) ? v + fac : v;
+        }
+#pragma unroll
+        for (int k = 0; k < kNeqs - 1; ++k) {
+            if (k >= j) break;
+            if ((sing >> k) & 1u) continue;
+            swap_rows(col, k, piv_get(piv, k));
+            const double t = col[k];
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+                if (i <= k) continue;
+                const double m = (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
+                col[i] += t * m;
+            }
+        }
+        if (j < kNeqs - 1) {
+            int p = j;
+            double best = fabs(col[j]);
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+                if (i <= j) continue;
+                const double v = fabs(col[i]);
+                if (v > best) {
+                    best = v;
+                    p = i;
+                }
+            }
+            piv |= (uint64_t)p << (4 * j);
+            double pv = col[j];
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i)
+                if (i > j) pv = (p == i) ? col[i] : pv;
+            if (pv == 0.0) {
+                info = j + 1;
+                sing |= 1u << j;
+            } else {
+                swap_rows(col, j, p);
+                const double mult = -1.0 / col[j];
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i)
+                    if (i > j) col[i] *= mult;
+            }
+            if (j < kCache) {
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) mc[j < kCache ? j : 0][i] = col[i];
+            }
+        } else {
+            piv |= (uint64_t)(kNeqs - 1) << (4 * j);
+            if (col[kNeqs - 1] == 0.0) info = kNeqs;
+        }
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) WR_AT(LU, i * kNeqs + j, st) = col[i];
+    }
+    return info;
+}
+
+// lu_solve (reproducer.cpp:154-181) from the workspace factors, x in registers.
+__device__ __forceinline__ void lu_solve_ws(const ThreadWs& w, uint64_t piv, double (&x)[kNeqs]) {
+    const int st = w.stride;
+    const double* __restrict__ LU = w.LU;
+#pragma unroll
+    for (int k = 0; k < kNeqs - 1; ++k) {
+        swap_rows(x, k, piv_get(piv, k));
+        const double t = x[k];
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j)
+            if (j > k) x[j] += t * WR_AT(LU, j * kNeqs + k, st);
+    }
+#pragma unroll
+    for (int kb = 0; kb < kNeqs; ++kb) {
+        const int k = kNeqs - 1 - kb;
+        x[k] /= WR_AT(LU, k * kNeqs + k, st);
+        const double t = -x[k];
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j)
+            if (j < k) x[j] += t * WR_AT(LU, j * kNeqs + k, st);
     }
 }
 

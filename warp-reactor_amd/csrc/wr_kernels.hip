@@ -70,6 +70,8 @@ struct Ctl {
     int step0;          // step index at the start of the run
     int hist_cap;       // capacity of dt_hist
     double* dt_hist;    // dt_grid of every outer step of the run (device)
+    int queue;          // next cell to hand out in the queue-fed advance
+    int pad2_;
 };
 
 constexpr int kPrepBlock = 256;
@@ -250,6 +252,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(Ctl* ctl, const double* par
     ctl->next_grid_time = next;
     ctl->limiting_cell = cell;
     ctl->integrated = 0;
+    ctl->queue = 0;
     const int k = ctl->step - ctl->step0;
     if (ctl->dt_hist && k >= 0 && k < ctl->hist_cap) ctl->dt_hist[k] = dt;
 }
@@ -324,6 +327,243 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance(DevCells s, DevWs ws, Ctl
     s.time[c] = ctl->next_grid_time;
     s.completed[c] = completed + 1;
     atomicAdd(&ctl->integrated, 1);
+}
+
+// ---- queue-fed advance ---------------------------------------------------------------
+// The same per-cell work as k_advance (reproducer.cpp:7736-7785 with
+// RODAS::integrate 7205-7347), reorganised for the GPU:
+//  * a fixed grid of persistent threads takes cells from an atomic queue;
+//  * the integrator is a flat state machine -- one loop iteration is one
+//    ROS2S attempt (or one singular-matrix retry) of the cell a lane holds --
+//    so a lane whose cell finishes takes the next cell at the next iteration
+//    instead of idling until its wave's slowest cell is done;
+//  * J, LU and the stage vectors live in a per-thread SoA workspace; the
+//    network code is register-budgeted (staging slots in the same workspace),
+//    so nothing is left to compiler spilling.
+// Counters, results and every arithmetic operation are those of the
+// reference; only where values are kept differs.
+constexpr int kQBlock = 256;
+constexpr int kMaxSteps = 10000000;  // configure_ros2s (reproducer.cpp:7589)
+
+// The per-thread workspace is the lane's private memory: the hardware
+// interleaves private (scratch) memory by lane, so element e of all 64 lanes
+// of a wave is one contiguous 512-byte line, and an element is addressed by
+// an immediate offset.  `zero` (always 0) is an index the compiler cannot
+// resolve, which keeps the array in memory instead of registers.
+constexpr int kPrivLen = 2 * kNeqs * kNeqs + kVecLen + kStageSlots;
+
+__global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero) {
+    using C = Ros2sCoef;
+    if (ctl->done) return;
+    double priv[kPrivLen];
+    double* const p = priv + zero;
+    ThreadWs w;
+    w.J = p;
+    w.LU = p + kNeqs * kNeqs;
+    w.vec = p + 2 * kNeqs * kNeqs;
+    w.stage = p + 2 * kNeqs * kNeqs + kVecLen;
+    w.stride = 1;
+    const int st = 1;
+    const int completed = ctl->completed;
+    const double dt_grid = ctl->dt_grid;
+    const double next_time = ctl->next_grid_time;
+    int c = -1;
+    double x = 0.0, h = 0.0, hopt = 0.0, hacc = 0.0, erracc = 1.0, rho = 0.0, e0 = 0.0;
+    int nsing = 0, n_step = 0, n_accept = 0;
+    bool reject = false, last = false, top = true;
+    Counters cnt{0, 0, 0, 0, 0, 0, 0};
+    uint64_t piv = 0;
+    for (;;) {
+        if (c < 0) {
+            // ---- take the next cell: advance_collapse_gridwide_kernel's preamble
+            c = atomicAdd(&ctl->queue, 1);
+            if (c >= s.n || ctl->failure != kSuccess) break;
+            if (s.completed[c] != completed) {
+                c = -1;
+                continue;
+            }
+            double xn[kNumSpec];
+#pragma unroll
+            for (int n = 0; n < kNumSpec; ++n) xn[n] = s.xn[n * s.cap + c];
+            const double local_dt = collapse_timestep(xn);
+            if (local_dt < 10.0) {
+                c = -1;
+                continue;
+            }
+            const double old_density = s.driver[c];
+            const double tff = local_dt / 1.0e-1;
+            const double driver = old_density + dt_grid * (old_density / tff);
+            s.driver[c] = driver;
+            if (!valid_positive(driver) || driver > 2.0e-6) {
+                c = -1;
+                continue;
+            }
+            const double ratio = driver / old_density;
+#pragma unroll
+            for (int n = 0; n < kNumSpec; ++n) xn[n] *= ratio;
+            rho = s.rho[c] * ratio;
+            s.rho[c] = rho;
+            // burn_ros2s (7592-7606): eos_rt, y = (xn, e)
+            e0 = eos_energy(xn, s.T[c]);
+#pragma unroll
+            for (int n = 0; n < kNumSpec; ++n) WR_AT(w.vec, kVecY + n, st) = xn[n];
+            WR_AT(w.vec, kVecY + kNumSpec, st) = e0;
+            // integrate() prologue, t = 0, tout = dt = dt_grid (> 0, checked by k_plan)
+            h = dt_grid;
+            if (fabs(h) <= 10.0 * kUround) h = 1.0e-6;
+            h = rmin(fabs(h), dt_grid - 0.0);
+            x = 0.0;
+            hopt = h;
+            hacc = 0.0;
+            erracc = 1.0;
+            nsing = 0;
+            n_step = 0;
+            n_accept = 0;
+            reject = false;
+            last = false;
+            top = true;
+            cnt = Counters{0, 0, 0, 0, 0, 0, 0};
+        }
+        int result = 0;  // 0: the cell goes on
+        if (top) {
+            // head of the outer loop of integrate (7236-7258)
+            if (n_step > kMaxSteps) {
+                result = kTooManySteps;
+            } else if (0.1 * fabs(h) <= fabs(x) * kUround) {
+                result = kDtUnderflow;
+            } else if (last) {
+                result = kSuccess;
+            } else {
+                hopt = h;
+                if (x + h * 1.0001 >= dt_grid) {
+                    h = dt_grid - x;
+                    last = true;
+                }
+                ws_jac_rhs(w);
+                cnt.n_jac += 1;
+                top = false;
+            }
+        }
+        if (result == 0) {
+            // retry loop body (7260-7345): factorize, three stages, controller
+            const double fac = 1.0 / (h * C::gamma);
+            if (lu_left(w, fac, piv) != 0) {
+                nsing += 1;
+                if (nsing >= 5) {
+                    result = kLuError;
+                } else {
+                    h *= 0.5;
+                    reject = true;
+                    last = false;
+                }
+            } else {
+                cnt.n_decomp += 1;
+                double v[kNeqs];
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) v[i] = WR_AT(w.vec, kVecF0 + i, st);
+                lu_solve_ws(w, piv, v);  // ak1
+                cnt.n_solve += 1;
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) {
+                    WR_AT(w.vec, kVecAk1 + i, st) = v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) = WR_AT(w.vec, kVecY + i, st) + C::a21 * v[i];
+                    WR_AT(w.vec, kVecAk2 + i, st) = (C::c21 / h) * v[i];
+                }
+                ws_rhs(w);
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i)
+                    v[i] = WR_AT(w.vec, kVecAk2 + i, st) + WR_AT(w.vec, kVecDy + i, st);
+                lu_solve_ws(w, piv, v);  // ak2
+                cnt.n_solve += 1;
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) {
+                    const double a1 = WR_AT(w.vec, kVecAk1 + i, st);
+                    WR_AT(w.vec, kVecAk2 + i, st) = v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) =
+                        WR_AT(w.vec, kVecY + i, st) + C::a31 * a1 + C::a32 * v[i];
+                    WR_AT(w.vec, kVecAk3 + i, st) = (C::c31 * a1 + C::c32 * v[i]) / h;
+                }
+                ws_rhs(w);
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i)
+                    v[i] = WR_AT(w.vec, kVecAk3 + i, st) + WR_AT(w.vec, kVecDy + i, st);
+                lu_solve_ws(w, piv, v);  // ak3
+                cnt.n_solve += 1;
+                double err = 0.0;
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) {
+                    const double a1 = WR_AT(w.vec, kVecAk1 + i, st);
+                    const double a2 = WR_AT(w.vec, kVecAk2 + i, st);
+                    const double yi = WR_AT(w.vec, kVecY + i, st);
+                    const double yn = yi + C::b1 * a1 + C::b2 * a2 + C::b3 * v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) = yn;
+                    const double wv = C::e1 * a1 + C::e2 * a2 + C::e3 * v[i];
+                    const double sk = atol_of(i) + rtol_of(i) * rmax(fabs(yi), fabs(yn));
+                    const double term = wv / sk;
+                    err += term * term;
+                }
+                cnt.n_rhs += 3;
+                n_step += 1;
+                cnt.n_step = n_step;
+                err = sqrt(err / static_cast<double>(kNeqs));
+                const double raw_fac = WR_CBRT(err) / kSafe;
+                const double fac_step = rmax(1.0 / kFacMax, rmin(1.0 / kFacMin, raw_fac));
+                double hnew = h / fac_step;
+                if (err <= 1.0) {
+                    n_accept += 1;
+                    cnt.n_accept = n_accept;
+                    if (n_accept > 1) {
+                        const double facgus = rmax(
+                            1.0 / kFacMax,
+                            rmin(1.0 / kFacMin, (hacc / h) * WR_CBRT((err * err) / erracc) / kSafe));
+                        hnew = h / rmax(fac_step, facgus);
+                    }
+                    hacc = h;
+                    erracc = rmax(1.0e-2, err);
+#pragma unroll
+                    for (int i = 0; i < kNeqs; ++i)
+                        WR_AT(w.vec, kVecY + i, st) = WR_AT(w.vec, kVecYnew + i, st);
+                    x += h;
+                    if (fabs(hnew) > dt_grid) hnew = dt_grid;
+                    if (reject) hnew = rmin(fabs(hnew), fabs(h));
+                    reject = false;
+                    h = hnew;
+                    top = true;
+                } else {
+                    reject = true;
+                    last = false;
+                    h = hnew;
+                    if (n_accept >= 1) cnt.n_reject += 1;
+                }
+            }
+        }
+        if (result != 0) {
+            // ---- the cell is done: burn_ros2s epilogue and the kernel's commit
+            const int cv[7] = {cnt.n_step, cnt.n_rhs, cnt.n_jac, cnt.n_decomp, cnt.n_solve,
+                               cnt.n_accept, cnt.n_reject};
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                s.stats[k * s.cap + c] += static_cast<unsigned long long>(cv[k] > 0 ? cv[k] : 0);
+            if (result == kSuccess) {
+                double xn[kNumSpec];
+#pragma unroll
+                for (int n = 0; n < kNumSpec; ++n) xn[n] = WR_AT(w.vec, kVecY + n, st);
+                const double e = WR_AT(w.vec, kVecY + kNumSpec, st);
+                const double T = post_step(xn, rho, e);
+#pragma unroll
+                for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
+                s.e[c] = e;
+                s.T[c] = T;
+                s.time[c] = next_time;
+                s.completed[c] = completed + 1;
+                atomicAdd(&ctl->integrated, 1);
+            } else {
+                s.e[c] = e0;  // eos_rt ran before the failed burn
+                set_failure(ctl, result, c);
+            }
+            c = -1;
+        }
+    }
 }
 
 __global__ void k_commit(Ctl* ctl) {
@@ -500,6 +740,9 @@ struct wr_context {
     wr_collapse_state* staging = nullptr;  // device AoS staging buffer
     void* cells_mem = nullptr;
     void* ws_mem = nullptr;
+    // advance kernel: 1 (default) queue-fed persistent kernel, 0 one thread per cell
+    int advance_kernel = 1;
+    int q_blocks = 0;
     // HIP-event timing of the advance launches of wr_run (when enabled)
     static constexpr int kChunk = 8;
     bool timing = false;
@@ -667,9 +910,36 @@ static int launch_prepare(wr_context* ctx) {
     return WR_SUCCESS;
 }
 
+// Workspace of the queue-fed kernel, sized for the persistent grid: the
+// resident blocks of k_advance_q on every CU (at most the blocks that the
+// cells fill).
+// Persistent grid of the queue-fed kernel: its resident blocks on every CU
+// (at most the blocks the cells fill).
+static int queue_grid(wr_context* ctx) {
+    if (ctx->q_blocks > 0) return WR_SUCCESS;
+    int cus = 0, per_cu = 0;
+    WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+           "hipDeviceGetAttribute(CU count)");
+    WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_advance_q, kQBlock, 0),
+           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (per_cu < 1) per_cu = 1;
+    int blocks = cus * per_cu;
+    const int need = blocks_for(ctx->cap, kQBlock);
+    if (blocks > need) blocks = need;
+    ctx->q_blocks = blocks < 1 ? 1 : blocks;
+    return WR_SUCCESS;
+}
+
 static int launch_advance(wr_context* ctx) {
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
-                       ctx->stream, ctx->cells, ctx->ws, ctx->ctl);
+    if (ctx->advance_kernel == 1) {
+        const int st = queue_grid(ctx);
+        if (st != WR_SUCCESS) return st;
+        hipLaunchKernelGGL(k_advance_q, dim3(ctx->q_blocks), dim3(kQBlock), 0, ctx->stream,
+                           ctx->cells, ctx->ctl, 0);
+    } else {
+        hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
+                           ctx->stream, ctx->cells, ctx->ws, ctx->ctl);
+    }
     WR_HIP(hipGetLastError(), "advance_collapse_gridwide_kernel launch");
     return WR_SUCCESS;
 }
@@ -755,6 +1025,13 @@ int wr_dt_history(const wr_context* ctx, double* out, int max_entries, int* coun
         WR_HIP(hipMemcpy(out, ctx->dt_hist, n * sizeof(double), hipMemcpyDeviceToHost),
                "hipMemcpy(dt history)");
     }
+    return WR_SUCCESS;
+}
+
+int wr_set_advance_kernel(wr_context* ctx, int kernel) {
+    if (!ctx || kernel < 0 || kernel > 1)
+        return fail(WR_ERR_ARGUMENT, "wr_set_advance_kernel: kernel must be 0 or 1");
+    ctx->advance_kernel = kernel;
     return WR_SUCCESS;
 }
 

@@ -144,7 +144,31 @@ class _Emitter:
         return "\n".join(lines), counts
 
 
-def emit_header(ir_path, out_path, counts_path=None):
+def emit_staged_pair(d, f_roots, j_roots, r_fused, r_rhs):
+    """Register-budgeted versions (stage.py): outputs leave through
+    CHEM_OUT(J, r, c, v) / CHEM_OUT(F, i, v) as soon as computed; values
+    that do not fit R doubles go through STAGE_STORE/STAGE_LOAD slots."""
+    import stage
+    outs = {}
+    for i, nid in enumerate(f_roots):
+        outs.setdefault(nid, []).append(f"F, {i}")
+    rhs_outs = {k: list(v) for k, v in outs.items()}
+    for (r, c), nid in j_roots.items():
+        outs.setdefault(nid, []).append(f"J, {r}, {c}")
+    fused, pf = stage.emit_staged(d, "chem_fused_staged", outs, r_fused, " CHEM_STAGE_PARAMS", "CHEM_OUT")
+    rhs, pr = stage.emit_staged(d, "chem_rhs_staged", rhs_outs, r_rhs, " CHEM_STAGE_PARAMS", "CHEM_OUT")
+    info = {"fused": {"R": r_fused, "slots": pf.nslots, "stores": pf.stores, "loads": pf.loads},
+            "rhs": {"R": r_rhs, "slots": pr.nslots, "stores": pr.stores, "loads": pr.loads}}
+    text = "\n".join([
+        "#ifdef CHEM_STAGE_PARAMS",
+        f"#define CHEM_FUSED_NSLOT {max(pf.nslots, 1)}",
+        f"#define CHEM_RHS_NSLOT {max(pr.nslots, 1)}",
+        fused, "", rhs,
+        "#endif /* CHEM_STAGE_PARAMS */", ""])
+    return text, info
+
+
+def emit_header(ir_path, out_path, counts_path=None, r_fused=96, r_rhs=96):
     d = specialise(dagmod.read_ir(ir_path))
     f_roots = [d.outputs[("ydot", i)] for i in range(dagmod.NUM_SPEC)] + [d.outputs[("edot",)]]
     j_roots = {(r, c): d.outputs[("jac", r, c)] for r in range(dagmod.NEQS) for c in range(dagmod.NEQS)}
@@ -152,6 +176,7 @@ def emit_header(ir_path, out_path, counts_path=None):
     fused, c_fused = em.function("chem_fused", f_roots, j_roots)
     em = _Emitter(d)
     rhs, c_rhs = em.function("chem_rhs", f_roots, None)
+    staged, staged_info = emit_staged_pair(d, f_roots, j_roots, r_fused, r_rhs)
     sha = d.meta.get("source_sha256", "?")
     hdr = [
         "/* Generated by warp-reactor_amd/gen/emit.py from primordial_network.ir",
@@ -177,6 +202,7 @@ def emit_header(ir_path, out_path, counts_path=None):
         "",
         rhs,
         "",
+        staged,
     ]
     text = "\n".join(hdr)
     os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
@@ -186,7 +212,7 @@ def emit_header(ir_path, out_path, counts_path=None):
     os.replace(tmp, out_path)
     if counts_path:
         with open(counts_path, "w") as f:
-            json.dump({"chem_fused": c_fused, "chem_rhs": c_rhs,
+            json.dump({"chem_fused": c_fused, "chem_rhs": c_rhs, "staged": staged_info,
                        "source_sha256": sha}, f, indent=1, sort_keys=True)
     return c_fused, c_rhs
 

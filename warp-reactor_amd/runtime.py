@@ -78,6 +78,7 @@ _SIGS = {
     "wr_prepare": (_I, [_VP, _I, _D, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
     "wr_dt_history": (_I, [_VP, _VP, _I, ctypes.POINTER(_I)]),
+    "wr_set_advance_kernel": (_I, [_VP, _I]),
     "wr_set_timing": (_I, [_VP, _I]),
     "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_eval_rhs": (_I, [_VP, _VP, _I, _VP]),
@@ -218,6 +219,10 @@ class Context:
         _check(self._lib.wr_dt_history(self._h, _ptr(out), len(out), ctypes.byref(n)),
                "wr_dt_history")
         return out[:n.value].copy()
+
+    def set_advance_kernel(self, kernel):
+        """1: queue-fed persistent kernel (default); 0: one thread per cell."""
+        _check(self._lib.wr_set_advance_kernel(self._h, int(kernel)), "wr_set_advance_kernel")
 
     def set_timing(self, enable=True):
         _check(self._lib.wr_set_timing(self._h, int(bool(enable))), "wr_set_timing")
