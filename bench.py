@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--max-outer", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--advance-kernel", type=int, default=1,
+                    help="1: queue-fed persistent kernel (default); 0: one thread per cell")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -107,6 +109,7 @@ def main():
     n = args.grid ** 3
     init = wr.make_collapse_states(n)
     ctx = wr.Context(n)
+    ctx.set_advance_kernel(args.advance_kernel)
     # prime: load the kernels with one outer step (not timed, not counted)
     ctx.upload(init)
     ctx.run(perturb=True, max_steps=1)
@@ -162,7 +165,8 @@ def main():
         "roofline": {"bound": "mfma", "compute_pipe": "VALU FP64 (MI355X FP64 peak 78.6 TFLOP/s, "
                      "vector = matrix)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
-                     "traffic": None, "kernel": "k_advance",
+                     "traffic": None,
+                     "kernel": "k_advance_q" if args.advance_kernel == 1 else "k_advance",
                      "mean_launch_ms": mean_launch_s * 1e3,
                      "flops_per_launch": per_launch_flops},
         "counters": counters,

@@ -146,7 +146,8 @@ int wr_run(wr_context *ctx, int perturb, int max_steps, int completed_global_ste
 int wr_run_info_get(const wr_context *ctx, wr_run_info *out);
 /* dt_grid of each outer step of the last wr_run (the host loop's dt_grid,
  * reproducer.cpp:7888-7897), in step order: copies min(count, max_entries)
- * values, *count = number of steps recorded (at most 4096).  With the initial
+ * values, *count = number of steps recorded (at most 4096; a last step that
+ * planned a dt but integrated no cell is recorded too).  With the initial
  * states this determines every cell's trajectory independently of the others,
  * so single cells can be replayed and checked. */
 int wr_dt_history(const wr_context *ctx, double *out, int max_entries, int *count);

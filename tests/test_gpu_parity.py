@@ -194,7 +194,10 @@ def test_advance_kernels_bit_exact(wr, oracle_wr, kernel, grid, perturb):
     ocells = oracle_wr.make_cells(n, wr.CELL_DTYPE)
     orr, ocompleted = oracle_wr.run_gridwide(ocells, perturb)
     assert info["status"] == orr == 1
-    assert info["completed_global_steps"] == ocompleted == len(dts)
+    assert info["completed_global_steps"] == ocompleted
+    # the history also holds the dt of a final step that integrated no cell
+    extra = 1 if info["stop_reason"] == 3 else 0
+    assert len(dts) == ocompleted + extra
     assert _cells_equal(cells, ocells)
 
 

@@ -263,3 +263,29 @@ def test_reference_cli_grid1_prints_438(tmp_path):
                          cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert out.returncode == 0
     assert "completed global collapse steps: 438" in out.stdout
+
+
+def test_staged_network_is_bit_identical(oracle):
+    """The register-budgeted network code (evaluation order rescheduled, values
+    parked in staging slots) performs the same operations on the same operands
+    as the plain generated code, so J and f agree bit for bit."""
+    import ctypes
+    lib = oracle.lib
+    vp = ctypes.c_void_p
+    lib.or_jac_rhs_staged.argtypes = [vp, vp, vp]
+    lib.or_rhs_staged.argtypes = [vp, vp]
+    lib.or_jac_rhs_staged.restype = None
+    lib.or_rhs_staged.restype = None
+    n = 0
+    for y in random_states(64, oracle, seed=21):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        j = np.zeros((15, 15))
+        f = np.zeros(15)
+        f2 = np.zeros(15)
+        lib.or_jac_rhs_staged(y.ctypes.data_as(vp), j.ctypes.data_as(vp), f.ctypes.data_as(vp))
+        lib.or_rhs_staged(y.ctypes.data_as(vp), f2.ctypes.data_as(vp))
+        assert np.array_equal(bits(j), bits(oracle.jac(y)))
+        assert np.array_equal(bits(f), bits(oracle.rhs(y)))
+        assert np.array_equal(bits(f2), bits(f))
+        n += 1
+    assert n == 64
