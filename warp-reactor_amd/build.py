@@ -27,10 +27,12 @@ ARCH = "gfx950"
 
 # -ffp-contract=off: every product and sum is rounded separately, in the
 # reference's order.  Packed-FP32 instructions are disabled for all HIP code.
+# -disable-machine-licm: the advance kernel's loop body holds the straight-line
+# network code; hoisting its constants out of the loop only spills them.
 HIP_FLAGS = [
     "-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=off",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument",
+    "-mllvm", "-disable-machine-licm", "-Wno-unused-command-line-argument",
 ]
 
 

@@ -357,10 +357,17 @@ constexpr int kStageSlots = (CHEM_FUSED_NSLOT > CHEM_RHS_NSLOT) ? CHEM_FUSED_NSL
 #define WR_AT(base, e, stride) ((base)[(e) * (stride)])
 
 // J(y) and f(y) with the register-budgeted network code; y is read from and
-// J, f0 written to the workspace.  Not inlined: inside a persistent loop the
-// compiler would hoist the network's ~1100 FP64 constants out of the loop and
-// spill them (over 1700 SGPR spills).
-__device__ __noinline__ void ws_jac_rhs(const ThreadWs& w) {
+// J, f0 written to the workspace.  Inlined into the advance kernel; the build
+// turns off machine-level loop-invariant code motion, which would otherwise
+// hoist the network's ~1100 FP64 constants out of the persistent loop and
+// spill them (over 1700 SGPR spills).  -DWR_NET_NOINLINE keeps the two
+// network wrappers as separate functions instead.
+#ifdef WR_NET_NOINLINE
+#define WR_NET_FN __device__ __noinline__
+#else
+#define WR_NET_FN __device__ __forceinline__
+#endif
+WR_NET_FN void ws_jac_rhs(const ThreadWs& w) {
     double X[kNumSpec];
     double yy[kNeqs];
 #pragma unroll
@@ -374,7 +381,7 @@ __device__ __noinline__ void ws_jac_rhs(const ThreadWs& w) {
 }
 
 // f(ynew) into dy.
-__device__ __noinline__ void ws_rhs(const ThreadWs& w) {
+WR_NET_FN void ws_rhs(const ThreadWs& w) {
     double X[kNumSpec];
     double yy[kNeqs];
 #pragma unroll
@@ -432,15 +439,17 @@ __device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& 
         }
 #pragma unroll
         for (int k = 0; k < kNeqs - 1; ++k) {
-            if (k >= j) break;
-            if ((sing >> k) & 1u) continue;
-            swap_rows(col, k, piv_get(piv, k));
-            const double t = col[k];
+            if (k < j && !((sing >> k) & 1u)) {
+                swap_rows(col, k, piv_get(piv, k));
+                const double t = col[k];
 #pragma unroll
-            for (int i = 0; i < kNeqs; ++i) {
-                if (i <= k) continue;
-                const double m = (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
-                col[i] += t * m;
+                for (int i = 0; i < kNeqs; ++i) {
+                    if (i > k) {
+                        const double m =
+                            (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
+                        col[i] += t * m;
+                    }
+                }
             }
         }
         if (j < kNeqs - 1) {

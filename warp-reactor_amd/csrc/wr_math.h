@@ -53,48 +53,45 @@ WR_MATH_FN double wr_from_bits(uint64_t b) {
 }
 
 WR_MATH_FN double wr_exp(double x) {
-    if (x != x) return x;
-    if (x > 709.79) return HUGE_VAL;   /* above ln(DBL_MAX): overflow */
-    if (x < -745.2) return 0.0;        /* below ln(min subnormal / 2) */
-    const double kd = rint(x * 0x1.71547652b82fep+0); /* x / ln2 */
-    double r = fma(-kd, 0x1.62e42fefa39efp-1, x);     /* ln2 high part */
-    r = fma(-kd, 0x1.abc9e3b39803fp-56, r);           /* ln2 low part */
-    double p = 0x1.6124613a86d09p-33;                 /* 1/13! */
-    p = fma(p, r, 0x1.1eed8eff8d898p-29);             /* 1/12! */
-    p = fma(p, r, 0x1.ae64567f544e4p-26);             /* 1/11! */
-    p = fma(p, r, 0x1.27e4fb7789f5cp-22);             /* 1/10! */
-    p = fma(p, r, 0x1.71de3a556c734p-19);             /* 1/9! */
-    p = fma(p, r, 0x1.a01a01a01a01ap-16);             /* 1/8! */
-    p = fma(p, r, 0x1.a01a01a01a01ap-13);             /* 1/7! */
-    p = fma(p, r, 0x1.6c16c16c16c17p-10);             /* 1/6! */
-    p = fma(p, r, 0x1.1111111111111p-7);              /* 1/5! */
-    p = fma(p, r, 0x1.5555555555555p-5);              /* 1/4! */
-    p = fma(p, r, 0x1.5555555555555p-3);              /* 1/3! */
-    p = fma(p, r, 0x1.0000000000000p-1);              /* 1/2! */
+    /* Branch-free: the polynomial path runs on x clamped to a safe range and
+     * the special cases (NaN, overflow above ln(DBL_MAX), underflow below
+     * ln(min subnormal / 2)) are selected afterwards. */
+    const double xc = fmin(fmax(x, -746.0), 710.0);
+    const double kd = rint(xc * 0x1.71547652b82fep+0); /* x / ln2 */
+    double r = fma(-kd, 0x1.62e42fefa39efp-1, xc);     /* ln2 high part */
+    r = fma(-kd, 0x1.abc9e3b39803fp-56, r);            /* ln2 low part */
+    double p = 0x1.6124613a86d09p-33;                  /* 1/13! */
+    p = fma(p, r, 0x1.1eed8eff8d898p-29);              /* 1/12! */
+    p = fma(p, r, 0x1.ae64567f544e4p-26);              /* 1/11! */
+    p = fma(p, r, 0x1.27e4fb7789f5cp-22);              /* 1/10! */
+    p = fma(p, r, 0x1.71de3a556c734p-19);              /* 1/9! */
+    p = fma(p, r, 0x1.a01a01a01a01ap-16);              /* 1/8! */
+    p = fma(p, r, 0x1.a01a01a01a01ap-13);              /* 1/7! */
+    p = fma(p, r, 0x1.6c16c16c16c17p-10);              /* 1/6! */
+    p = fma(p, r, 0x1.1111111111111p-7);               /* 1/5! */
+    p = fma(p, r, 0x1.5555555555555p-5);               /* 1/4! */
+    p = fma(p, r, 0x1.5555555555555p-3);               /* 1/3! */
+    p = fma(p, r, 0x1.0000000000000p-1);               /* 1/2! */
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    return ldexp(p, (int)kd);
+    double res = ldexp(p, (int)kd);
+    res = (x > 709.79) ? HUGE_VAL : res;
+    res = (x < -745.2) ? 0.0 : res;
+    return (x != x) ? x : res;
 }
 
 WR_MATH_FN double wr_log(double x) {
-    if (!(x > 0.0) || x == HUGE_VAL) {
-        if (x != x) return x;
-        if (x == 0.0) return -HUGE_VAL;
-        if (x < 0.0) return (x - x) / (x - x); /* NaN */
-        return x;                               /* +inf */
-    }
-    int e_adj = 0;
-    if (x < 0x1p-1022) { /* subnormal: normalise */
-        x *= 0x1p54;
-        e_adj = -54;
-    }
-    const uint64_t b = wr_bits_of(x);
-    int e = (int)(b >> 52) - 1023 + e_adj;
+    /* Branch-free: the series runs for every x (subnormals scaled by 2^54
+     * first) and the special cases are selected afterwards: NaN -> x,
+     * +-0 -> -inf, x < 0 -> NaN, +inf -> +inf. */
+    const int sub = x < 0x1p-1022;
+    const double xs = sub ? x * 0x1p54 : x;
+    const uint64_t b = wr_bits_of(xs);
+    int e = (int)((b >> 52) & 0x7ff) - 1023 - (sub ? 54 : 0);
     double m = wr_from_bits((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-    if (m > 0x1.6a09e667f3bcdp+0) { /* sqrt(2) */
-        m *= 0.5;
-        e += 1;
-    }
+    const int big = m > 0x1.6a09e667f3bcdp+0; /* sqrt(2) */
+    m = big ? m * 0.5 : m;
+    e += big;
     const double f = m - 1.0;
     const double s = f / (2.0 + f);
     const double z = s * s;
@@ -113,8 +110,12 @@ WR_MATH_FN double wr_log(double x) {
     const double hfsq = 0.5 * f * f;
     const double dk = (double)e;
     /* ln2 split so that dk * high is exact for |e| < 2^11 */
-    return dk * 0x1.62e42fefa3800p-1 -
-           ((hfsq - (s * (hfsq + R) + dk * 0x1.ef35793c76730p-45)) - f);
+    double res = dk * 0x1.62e42fefa3800p-1 -
+                 ((hfsq - (s * (hfsq + R) + dk * 0x1.ef35793c76730p-45)) - f);
+    res = (x == HUGE_VAL) ? x : res;
+    res = (x < 0.0) ? (double)NAN : res;
+    res = (x == 0.0) ? -HUGE_VAL : res;
+    return (x != x) ? x : res;
 }
 
 WR_MATH_FN double wr_cbrt(double x) {

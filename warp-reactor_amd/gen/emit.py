@@ -144,10 +144,11 @@ class _Emitter:
         return "\n".join(lines), counts
 
 
-def emit_staged_pair(d, f_roots, j_roots, r_fused, r_rhs):
+def emit_staged_pair(d, f_roots, j_roots, r_fused, r_rhs, prefetch=500, prefetch_extra=32):
     """Register-budgeted versions (stage.py): outputs leave through
     CHEM_OUT(J, r, c, v) / CHEM_OUT(F, i, v) as soon as computed; values
-    that do not fit R doubles go through STAGE_STORE/STAGE_LOAD slots."""
+    that do not fit R doubles go through STAGE_STORE/STAGE_LOAD slots, and
+    each reload is issued up to `prefetch` operations before its use."""
     import stage
     outs = {}
     for i, nid in enumerate(f_roots):
@@ -155,10 +156,13 @@ def emit_staged_pair(d, f_roots, j_roots, r_fused, r_rhs):
     rhs_outs = {k: list(v) for k, v in outs.items()}
     for (r, c), nid in j_roots.items():
         outs.setdefault(nid, []).append(f"J, {r}, {c}")
-    fused, pf = stage.emit_staged(d, "chem_fused_staged", outs, r_fused, " CHEM_STAGE_PARAMS", "CHEM_OUT")
-    rhs, pr = stage.emit_staged(d, "chem_rhs_staged", rhs_outs, r_rhs, " CHEM_STAGE_PARAMS", "CHEM_OUT")
+    fused, pf = stage.emit_staged(d, "chem_fused_staged", outs, r_fused, " CHEM_STAGE_PARAMS", "CHEM_OUT",
+                                  prefetch=prefetch, prefetch_extra=prefetch_extra)
+    rhs, pr = stage.emit_staged(d, "chem_rhs_staged", rhs_outs, r_rhs, " CHEM_STAGE_PARAMS", "CHEM_OUT",
+                                prefetch=prefetch, prefetch_extra=prefetch_extra)
     info = {"fused": {"R": r_fused, "slots": pf.nslots, "stores": pf.stores, "loads": pf.loads},
-            "rhs": {"R": r_rhs, "slots": pr.nslots, "stores": pr.stores, "loads": pr.loads}}
+            "rhs": {"R": r_rhs, "slots": pr.nslots, "stores": pr.stores, "loads": pr.loads},
+            "prefetch": prefetch, "prefetch_extra": prefetch_extra}
     text = "\n".join([
         "#ifdef CHEM_STAGE_PARAMS",
         f"#define CHEM_FUSED_NSLOT {max(pf.nslots, 1)}",
@@ -168,7 +172,7 @@ def emit_staged_pair(d, f_roots, j_roots, r_fused, r_rhs):
     return text, info
 
 
-def emit_header(ir_path, out_path, counts_path=None, r_fused=96, r_rhs=96):
+def emit_header(ir_path, out_path, counts_path=None, r_fused=128, r_rhs=96):
     d = specialise(dagmod.read_ir(ir_path))
     f_roots = [d.outputs[("ydot", i)] for i in range(dagmod.NUM_SPEC)] + [d.outputs[("edot",)]]
     j_roots = {(r, c): d.outputs[("jac", r, c)] for r in range(dagmod.NEQS) for c in range(dagmod.NEQS)}

@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import bisect
 import collections
+import struct
 
 import dag as dagmod
 
@@ -113,6 +114,7 @@ class Plan:
         self.stores = 0
         self.loads = 0
         self.peak = 0
+        self.pressure = []  # values held in registers at each action
 
 
 def allocate(d, seq, R):
@@ -142,6 +144,7 @@ def allocate(d, seq, R):
                 plan.actions.append(("load", a, slot_of[a]))
                 plan.loads += 1
                 inreg.add(a)
+                plan.pressure.append(len(inreg))
         # make room: evict furthest next use, never the operands of this op
         while len(inreg) + 1 > R:
             cands = [v for v in inreg if v not in args]
@@ -155,10 +158,12 @@ def allocate(d, seq, R):
                 slot_of[far] = s
                 slot_owner[s] = far
                 plan.actions.append(("store", far, s))
+                plan.pressure.append(len(inreg))
                 plan.stores += 1
             inreg.discard(far)
         plan.actions.append(("op", n))
         inreg.add(n)
+        plan.pressure.append(len(inreg))
         plan.peak = max(plan.peak, len(inreg))
         # retire dead values (including this op if it has no later use)
         for v in args + [n]:
@@ -170,20 +175,81 @@ def allocate(d, seq, R):
     return plan
 
 
-def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT"):
+def hoist_loads(plan, distance, extra):
+    """Move every staging reload up to `distance` actions earlier (never above
+    the store that filled its slot), as long as no point then holds more than
+    R + `extra` values: the load's latency is then covered by the work in
+    between instead of stalling its first use.  Returns a new action list."""
+    import numpy as np
+    acts = plan.actions
+    n = len(acts)
+    press = np.array(plan.pressure, dtype=np.int64)
+    cap = int(press.max()) + extra
+    store_at = {}
+    for i, a in enumerate(acts):
+        if a[0] == "store":
+            store_at[(a[1], a[2])] = i
+    moved_to = {}
+    for i, a in enumerate(acts):
+        if a[0] != "load":
+            continue
+        lo = max(store_at[(a[1], a[2])] + 1, i - distance)
+        if lo >= i:
+            continue
+        seg = press[lo:i]
+        over = np.nonzero(seg + 1 > cap)[0]
+        q = lo + (int(over[-1]) + 1 if len(over) else 0)
+        if q >= i:
+            continue
+        press[q:i] += 1
+        moved_to[i] = q
+    at = collections.defaultdict(list)
+    for i, q in moved_to.items():
+        at[q].append(i)
+    out = []
+    for i, a in enumerate(acts):
+        for j in at.get(i, ()):
+            out.append(acts[j])
+        if i not in moved_to:
+            out.append(a)
+    assert len(out) == n
+    return out
+
+
+# FP64 constants the gfx950 VALU takes as inline operands (no literal needed)
+_INLINE_F64 = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
+
+
+def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT", prefetch=0,
+                prefetch_extra=0, const_table=None, sync_every=0):
+    """const_table: name of a table the constants are read from through
+    CHEM_K(name, i) (indices in order of first use; values in plan.ktab),
+    except those the hardware encodes inline.  sync_every: emit CHEM_SYNC()
+    every that many operations (keeps the waves of a block in lockstep so
+    they share instruction fetches)."""
     """outputs: dict node -> list of output keys (str) handed to OUT macro.
     Returns (source text, plan)."""
     roots = list(outputs)
     seq = schedule(d, roots)
     plan = allocate(d, seq, R)
+    if prefetch > 0:
+        plan.actions = hoist_loads(plan, prefetch, prefetch_extra)
     cur = {}       # node -> current C name
     gen = collections.Counter()
     lines = [f"CHEM_FN void {fname}(double T, const double* X{sig_extra})", "{"]
+    kidx = {}
+    plan.ktab = []
 
     def ref(nid):
         n = d.nodes[nid]
         if n.op == "const":
             v = d.cval(nid)
+            if const_table and v == v and abs(v) != float("inf") and v not in _INLINE_F64:
+                bits_ = struct.pack("<d", v)
+                if bits_ not in kidx:
+                    kidx[bits_] = len(plan.ktab)
+                    plan.ktab.append(v)
+                return f"CHEM_K({const_table}, {kidx[bits_]})"
             s = v.hex() if v == v and abs(v) != float("inf") else (
                 "(0.0/0.0)" if v != v else ("(1.0/0.0)" if v > 0 else "(-1.0/0.0)"))
             return f"({s})" if s.startswith("-") else s
@@ -222,10 +288,14 @@ def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT"):
         if _leaf(d, nid):
             for k in keys:
                 lines.append(f"    {out_macro}({k}, {ref(nid)});")
+    nops = 0
     for act in plan.actions:
         kind, nid = act[0], act[1]
         is_b = d.is_bool(nid)
         if kind == "op":
+            nops += 1
+            if sync_every and nops % sync_every == 0:
+                lines.append("    CHEM_SYNC();")
             gen[nid] += 1
             nm = ("b" if is_b else "v") + f"{nid}"
             if gen[nid] > 1:
