@@ -15,7 +15,11 @@ the generated network and of the LU/solve/stage arithmetic, times the ROS2S
 counters of the run) per advance launch / its mean duration measured with HIP
 events on the launch stream.  `cpu_baseline` times the reference's own CPU
 build (oracle/_ref, single-threaded) on a bounded sample of the same
-workload.
+workload.  `roofline.traffic` is the HBM bytes per advance dispatch from
+the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated).
+`plain_fp64` times the same run with the thread-per-cell kernel (no queue,
+network left to the compiler's register allocation) and checks that the
+tuned default produces bit-identical final states.
 """
 import argparse
 import importlib
@@ -31,6 +35,19 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X vector FP64 (= FP64 matrix), dense
+TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "k_advance_q_64cube_pmc_traffic.txt")
+
+
+def measured_traffic(kernel, grid):
+    """HBM bytes per dispatch of the advance kernel from the committed counter
+    summary (64^3 default run only), or None."""
+    if kernel != "k_advance_q" or grid != 64 or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    with open(TRAFFIC_SUMMARY) as f:
+        for line in f:
+            if line.startswith("k_advance_q_hbm_bytes_per_dispatch:"):
+                return float(line.split(":")[1])
+    return None
 
 
 def network_op_counts():
@@ -99,6 +116,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--advance-kernel", type=int, default=1,
                     help="1: queue-fed persistent kernel (default); 0: one thread per cell")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the thread-per-cell comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -171,6 +190,30 @@ def main():
                      "flops_per_launch": per_launch_flops},
         "counters": counters,
     }
+    kname = line["roofline"]["kernel"]
+    traffic = measured_traffic(kname, args.grid)
+    line["roofline"]["traffic"] = traffic
+    if traffic and adv_launches:
+        line["roofline"]["traffic_gbs"] = traffic / mean_launch_s / 1e9
+    if not args.no_plain and args.advance_kernel == 1:
+        # the same run with the thread-per-cell kernel: time and bit identity
+        ctx.set_advance_kernel(0)
+        info0, el0 = one_run()
+        plain = np.zeros(n, dtype=wr.CELL_DTYPE)
+        ctx.download(plain)
+        mask = np.ones(216, bool)
+        mask[156:160] = False
+        same = bool(np.array_equal(cells.view(np.uint8).reshape(n, 216)[:, mask],
+                                   plain.view(np.uint8).reshape(n, 216)[:, mask]))
+        a, b = cells["current"]["xn"], plain["current"]["xn"]
+        floor = np.abs(b) > 1e-20
+        rel = np.abs(a - b)[floor] / np.abs(b)[floor]
+        plain_value = info0["cell_advances"] / el0
+        line["plain_fp64"] = {"kernel": "k_advance (one thread per cell)", "run_seconds": el0,
+                              "value": plain_value, "tuned_speedup": value / plain_value,
+                              "tuned_final_states_bit_identical": same,
+                              "tuned_max_species_rel_err": float(rel.max()) if rel.size else 0.0}
+        print(f"# plain run: {el0:.3f} s, bit-identical {same}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(line))

@@ -167,6 +167,11 @@ int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_tim
  * network code); 0 = one thread per cell.  Results are identical. */
 int wr_set_advance_kernel(wr_context *ctx, int kernel);
 
+/* Integrator step limit of the burns of subsequent runs (<= 0: 1e7, the
+ * configure_ros2s value, reproducer.cpp:7589).  Lower values force
+ * WR_TOO_MANY_STEPS failures, for testing the failure path. */
+int wr_set_integrator_max_steps(wr_context *ctx, int max_steps);
+
 /* Time the advance kernel launches of subsequent wr_run calls with HIP events
  * recorded on the context's stream (off by default). */
 int wr_set_timing(wr_context *ctx, int enable);

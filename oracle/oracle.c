@@ -428,6 +428,11 @@ done:
 
 static uint64_t clamp_u64(int v) { return v > 0 ? (uint64_t)v : 0u; }
 
+/* Integrator step limit of burn_ros2s: configure_ros2s sets 1e7
+ * (reproducer.cpp:7589); tests lower it to force TOO_MANY_STEPS. */
+static int g_integ_max_steps = 10000000;
+void or_set_integrator_max_steps(int m) { g_integ_max_steps = m > 0 ? m : 10000000; }
+
 /* burn_ros2s, reproducer.cpp:7592-7627 */
 int or_burn_ros2s(or_burn_t *s, double dt, or_stats *st) {
     double y[NEQ], t_end, dt_end;
@@ -435,7 +440,7 @@ int or_burn_ros2s(or_burn_t *s, double dt, or_stats *st) {
     eos_rt(s);
     for (int n = 0; n < NSPEC; ++n) y[n] = s->xn[n];
     y[NSPEC] = s->e;
-    const int r = or_integrate(y, dt, dt, 10000000, cnt, &t_end, &dt_end);
+    const int r = or_integrate(y, dt, dt, g_integ_max_steps, cnt, &t_end, &dt_end);
     st->internal_steps += clamp_u64(cnt[0]);
     st->rhs_calls += clamp_u64(cnt[1]);
     st->jacobian_calls += clamp_u64(cnt[2]);

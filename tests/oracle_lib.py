@@ -43,7 +43,8 @@ class _Cpu:
             "run_gridwide": (_I, [_VP, _I, _I, _I, ctypes.POINTER(_I)]),
         }
         if p == "or_":
-            sig.update({"libm_exp": (_D, [_D]), "libm_log": (_D, [_D]), "libm_cbrt": (_D, [_D]),
+            sig.update({"set_integrator_max_steps": (None, [_I]),
+                        "libm_exp": (_D, [_D]), "libm_log": (_D, [_D]), "libm_cbrt": (_D, [_D]),
                         "replay_cells": (None, [_VP, _VP, _I, _VP, _I, _I, _I, _D, _VP])})
         for name, (res, args) in sig.items():
             fn = getattr(self.lib, p + name)

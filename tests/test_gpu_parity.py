@@ -201,6 +201,35 @@ def test_advance_kernels_bit_exact(wr, oracle_wr, kernel, grid, perturb):
     assert _cells_equal(cells, ocells)
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_advance_failure_path_bit_exact(wr, oracle_wr, kernel):
+    """TOO_MANY_STEPS forced from the host (integrator limit 12 instead of
+    1e7): on a 3^3 grid one cell fails at outer step 232.  Every cell runs in
+    the failing step (27 cells start before any fails), so the grid matches
+    the oracle bit for bit, including the failed cell's state: density-scaled
+    composition, e from eos_rt, counters added (reproducer.cpp:7760-7784)."""
+    n = 27
+    cells = wr.make_collapse_states(n)
+    with wr.Context(n) as ctx:
+        ctx.set_advance_kernel(kernel)
+        ctx.set_integrator_max_steps(12)
+        ctx.upload(cells)
+        info = ctx.run(perturb=True)
+        ctx.download(cells)
+    oracle_wr._set_integrator_max_steps(12)
+    try:
+        ocells = oracle_wr.make_cells(n, wr.CELL_DTYPE)
+        orr, ocompleted = oracle_wr.run_gridwide(ocells, True)
+    finally:
+        oracle_wr._set_integrator_max_steps(0)
+    assert orr == info["status"] == -4
+    assert info["completed_global_steps"] == ocompleted == 232
+    assert info["stop_reason"] == 4 and info["failed_phase"] == 2
+    failed = np.nonzero(ocells["completed_steps"] == ocompleted)[0]
+    assert list(failed) == [info["failed_cell"]]
+    assert _cells_equal(cells, ocells)
+
+
 def test_advance_kernels_agree_16cube(wr):
     """16^3 cells, 300 outer steps: the queue-fed kernel (lanes refilled from
     an atomic queue, cells finishing in any order) and the thread-per-cell

@@ -39,6 +39,8 @@ struct DevCells {
     double* driver;
     int* completed;
     unsigned long long* stats;  // [7][cap]
+    int* cost;                  // ROS2S attempts of the cell's last advance
+    int* order;                 // queue position -> cell (most attempts first)
     int n;
     int cap;
 };
@@ -70,8 +72,10 @@ struct Ctl {
     int step0;          // step index at the start of the run
     int hist_cap;       // capacity of dt_hist
     double* dt_hist;    // dt_grid of every outer step of the run (device)
-    int queue;          // next cell to hand out in the queue-fed advance
-    int pad2_;
+    int queue;          // next queue position to hand out in the queue-fed advance
+    int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
+    int bin_count[16];  // cells per attempt-count bin, for the queue order
+    int bin_used[16];   // queue positions already handed to blocks, per bin
 };
 
 constexpr int kPrepBlock = 256;
@@ -130,6 +134,7 @@ __global__ void k_scatter(const wr_collapse_state* __restrict__ in, DevCells s) 
     s.time[c] = a.time;
     s.driver[c] = a.density_driver;
     s.completed[c] = a.completed_steps;
+    s.cost[c] = 0;
     const unsigned long long* st = reinterpret_cast<const unsigned long long*>(&a.stats);
 #pragma unroll
     for (int k = 0; k < 7; ++k) s.stats[k * s.cap + c] = st[k];
@@ -300,8 +305,9 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance(DevCells s, DevWs ws, Ctl
     y[kNumSpec] = e0;
     Counters cnt;
     double t_end, dt_end;
-    const int result = integrate_cell(y, dt_grid, dt_grid, 10000000, cnt, ws.J + c, ws.LU + c,
-                                       ws.cap, t_end, dt_end);
+    const int result = integrate_cell(y, dt_grid, dt_grid, ctl->integ_max_steps, cnt, ws.J + c,
+                                       ws.LU + c, ws.cap, t_end, dt_end);
+    s.cost[c] = cnt.n_step;
     const int cv[7] = {cnt.n_step, cnt.n_rhs, cnt.n_jac, cnt.n_decomp, cnt.n_solve,
                        cnt.n_accept, cnt.n_reject};
 #pragma unroll
@@ -327,6 +333,55 @@ __global__ __launch_bounds__(kAdvBlock) void k_advance(DevCells s, DevWs ws, Ctl
     s.time[c] = ctl->next_grid_time;
     s.completed[c] = completed + 1;
     atomicAdd(&ctl->integrated, 1);
+}
+
+// ---- queue order ------------------------------------------------------------------
+// Cells are handed out most-expensive first (longest-processing-time order):
+// the attempt count of a cell's last advance predicts its next one, and the
+// one or few cells that need tens of attempts then start at the beginning of
+// the step instead of finishing long after the rest.  Counting sort over 16
+// bins (15 = 15 or more attempts); the order within a bin is arbitrary.
+// Results do not depend on the order: every cell's advance is independent.
+constexpr int kCostBins = 16;
+constexpr int kOrderBlock = 256;
+
+__device__ __forceinline__ int cost_bin(int attempts) {
+    return attempts < 0 ? 0 : (attempts < kCostBins - 1 ? attempts : kCostBins - 1);
+}
+
+__global__ __launch_bounds__(kOrderBlock) void k_order_count(DevCells s, Ctl* ctl) {
+    __shared__ int h[kCostBins];
+    if (ctl->done) return;
+    if (threadIdx.x < kCostBins) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < s.n) atomicAdd(&h[cost_bin(s.cost[c])], 1);
+    __syncthreads();
+    if (threadIdx.x < kCostBins && h[threadIdx.x] != 0)
+        atomicAdd(&ctl->bin_count[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kOrderBlock) void k_order_scatter(DevCells s, Ctl* ctl) {
+    __shared__ int h[kCostBins];
+    __shared__ int base[kCostBins];
+    if (ctl->done) return;
+    if (threadIdx.x < kCostBins) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = 0, rank = 0;
+    if (c < s.n) {
+        b = cost_bin(s.cost[c]);
+        rank = atomicAdd(&h[b], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kCostBins) {
+        const int bb = threadIdx.x;
+        int start = 0;  // higher bins come first
+        for (int k = kCostBins - 1; k > bb; --k) start += ctl->bin_count[k];
+        base[bb] = (h[bb] != 0) ? start + atomicAdd(&ctl->bin_used[bb], h[bb]) : 0;
+    }
+    __syncthreads();
+    if (c < s.n) s.order[base[b] + rank] = c;
 }
 
 // ---- queue-fed advance ---------------------------------------------------------------
@@ -367,6 +422,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     const int completed = ctl->completed;
     const double dt_grid = ctl->dt_grid;
     const double next_time = ctl->next_grid_time;
+    const int max_steps = ctl->integ_max_steps;
     int c = -1;
     double x = 0.0, h = 0.0, hopt = 0.0, hacc = 0.0, erracc = 1.0, rho = 0.0, e0 = 0.0;
     int nsing = 0, n_step = 0, n_accept = 0;
@@ -376,8 +432,9 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     for (;;) {
         if (c < 0) {
             // ---- take the next cell: advance_collapse_gridwide_kernel's preamble
-            c = atomicAdd(&ctl->queue, 1);
-            if (c >= s.n || ctl->failure != kSuccess) break;
+            const int q = atomicAdd(&ctl->queue, 1);
+            if (q >= s.n || ctl->failure != kSuccess) break;
+            c = s.order[q];
             if (s.completed[c] != completed) {
                 c = -1;
                 continue;
@@ -400,7 +457,10 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
             }
             const double ratio = driver / old_density;
 #pragma unroll
-            for (int n = 0; n < kNumSpec; ++n) xn[n] *= ratio;
+            for (int n = 0; n < kNumSpec; ++n) {
+                xn[n] *= ratio;
+                s.xn[n * s.cap + c] = xn[n];  // the kernel scales the state in place
+            }
             rho = s.rho[c] * ratio;
             s.rho[c] = rho;
             // burn_ros2s (7592-7606): eos_rt, y = (xn, e)
@@ -427,7 +487,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
         int result = 0;  // 0: the cell goes on
         if (top) {
             // head of the outer loop of integrate (7236-7258)
-            if (n_step > kMaxSteps) {
+            if (n_step > max_steps) {
                 result = kTooManySteps;
             } else if (0.1 * fabs(h) <= fabs(x) * kUround) {
                 result = kDtUnderflow;
@@ -544,6 +604,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 #pragma unroll
             for (int k = 0; k < 7; ++k)
                 s.stats[k * s.cap + c] += static_cast<unsigned long long>(cv[k] > 0 ? cv[k] : 0);
+            s.cost[c] = cnt.n_step;
             if (result == kSuccess) {
                 double xn[kNumSpec];
 #pragma unroll
@@ -568,6 +629,10 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 
 __global__ void k_commit(Ctl* ctl) {
     if (ctl->done) return;
+    for (int b = 0; b < kCostBins; ++b) {
+        ctl->bin_count[b] = 0;
+        ctl->bin_used[b] = 0;
+    }
     if (ctl->failure != kSuccess) {
         ctl->done = 1;
         ctl->stop_reason = WR_STOP_FAILURE;
@@ -742,6 +807,7 @@ struct wr_context {
     void* ws_mem = nullptr;
     // advance kernel: 1 (default) queue-fed persistent kernel, 0 one thread per cell
     int advance_kernel = 1;
+    int integ_max_steps = kMaxSteps;
     int q_blocks = 0;
     // HIP-event timing of the advance launches of wr_run (when enabled)
     static constexpr int kChunk = 8;
@@ -793,7 +859,7 @@ int wr_context_create(int device, int capacity_cells, wr_context** out) {
     if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return cleanup(WR_ERR_HIP);
     if (!hip_ok(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate"))
         return cleanup(WR_ERR_HIP);
-    const size_t cell_bytes = cap * (sizeof(double) * (kNumSpec + 5) + sizeof(int) +
+    const size_t cell_bytes = cap * (sizeof(double) * (kNumSpec + 5) + sizeof(int) * 3 +
                                      sizeof(unsigned long long) * 7) + 4096;
     if (!hip_ok(hipMalloc(&ctx->cells_mem, cell_bytes), "hipMalloc(cells)"))
         return cleanup(WR_ERR_HIP);
@@ -811,6 +877,8 @@ int wr_context_create(int device, int capacity_cells, wr_context** out) {
     ctx->cells.driver = reinterpret_cast<double*>(take(cap * sizeof(double)));
     ctx->cells.stats = reinterpret_cast<unsigned long long*>(take(cap * 7 * sizeof(unsigned long long)));
     ctx->cells.completed = reinterpret_cast<int*>(take(cap * sizeof(int)));
+    ctx->cells.cost = reinterpret_cast<int*>(take(cap * sizeof(int)));
+    ctx->cells.order = reinterpret_cast<int*>(take(cap * sizeof(int)));
     ctx->cells.cap = ctx->cap;
     ctx->cells.n = 0;
     const size_t ws_bytes = cap * kNeqs * kNeqs * sizeof(double) * 2;
@@ -885,6 +953,7 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.stop_reason = (max_steps <= 0) ? WR_STOP_STEP_LIMIT : WR_STOP_NONE;
     h.failed_step = -1;
     h.limiting_cell = -1;
+    h.integ_max_steps = ctx->integ_max_steps;
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
            "hipMemcpy(control to device)");
@@ -934,6 +1003,11 @@ static int launch_advance(wr_context* ctx) {
     if (ctx->advance_kernel == 1) {
         const int st = queue_grid(ctx);
         if (st != WR_SUCCESS) return st;
+        const int nb = blocks_for(ctx->n, kOrderBlock);
+        hipLaunchKernelGGL(k_order_count, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
+                           ctx->ctl);
+        hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
+                           ctx->ctl);
         hipLaunchKernelGGL(k_advance_q, dim3(ctx->q_blocks), dim3(kQBlock), 0, ctx->stream,
                            ctx->cells, ctx->ctl, 0);
     } else {
@@ -1032,6 +1106,12 @@ int wr_set_advance_kernel(wr_context* ctx, int kernel) {
     if (!ctx || kernel < 0 || kernel > 1)
         return fail(WR_ERR_ARGUMENT, "wr_set_advance_kernel: kernel must be 0 or 1");
     ctx->advance_kernel = kernel;
+    return WR_SUCCESS;
+}
+
+int wr_set_integrator_max_steps(wr_context* ctx, int max_steps) {
+    if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_set_integrator_max_steps: null context");
+    ctx->integ_max_steps = max_steps > 0 ? max_steps : kMaxSteps;
     return WR_SUCCESS;
 }
 

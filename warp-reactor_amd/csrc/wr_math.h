@@ -53,9 +53,12 @@ WR_MATH_FN double wr_from_bits(uint64_t b) {
 }
 
 WR_MATH_FN double wr_exp(double x) {
-    /* Branch-free: the polynomial path runs on x clamped to a safe range and
-     * the special cases (NaN, overflow above ln(DBL_MAX), underflow below
-     * ln(min subnormal / 2)) are selected afterwards. */
+    /* Branch-free: the polynomial path runs on x clamped to [-746, 710].
+     * Above 709.79 (past ln(DBL_MAX)) ldexp then overflows to +inf, and below
+     * -745.2 (past ln of half the least subnormal) it rounds to +0, exactly
+     * the values the cut-offs of the definition return (checked for every
+     * double within 1e7 ulps of each cut-off and on a stride beyond), so
+     * only NaN needs a select. */
     const double xc = fmin(fmax(x, -746.0), 710.0);
     const double kd = rint(xc * 0x1.71547652b82fep+0); /* x / ln2 */
     double r = fma(-kd, 0x1.62e42fefa39efp-1, xc);     /* ln2 high part */
@@ -74,9 +77,7 @@ WR_MATH_FN double wr_exp(double x) {
     p = fma(p, r, 0x1.0000000000000p-1);               /* 1/2! */
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    double res = ldexp(p, (int)kd);
-    res = (x > 709.79) ? HUGE_VAL : res;
-    res = (x < -745.2) ? 0.0 : res;
+    const double res = ldexp(p, (int)kd);
     return (x != x) ? x : res;
 }
 

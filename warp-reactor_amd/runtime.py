@@ -80,6 +80,7 @@ _SIGS = {
     "wr_dt_history": (_I, [_VP, _VP, _I, ctypes.POINTER(_I)]),
     "wr_set_advance_kernel": (_I, [_VP, _I]),
     "wr_set_timing": (_I, [_VP, _I]),
+    "wr_set_integrator_max_steps": (_I, [_VP, _I]),
     "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_eval_rhs": (_I, [_VP, _VP, _I, _VP]),
     "wr_eval_math": (_I, [_VP, _I, _VP, _I, _VP]),
@@ -223,6 +224,11 @@ class Context:
     def set_advance_kernel(self, kernel):
         """1: queue-fed persistent kernel (default); 0: one thread per cell."""
         _check(self._lib.wr_set_advance_kernel(self._h, int(kernel)), "wr_set_advance_kernel")
+
+    def set_integrator_max_steps(self, max_steps):
+        """Integrator step limit of subsequent runs (<= 0: 1e7, the reference's)."""
+        _check(self._lib.wr_set_integrator_max_steps(self._h, int(max_steps)),
+               "wr_set_integrator_max_steps")
 
     def set_timing(self, enable=True):
         _check(self._lib.wr_set_timing(self._h, int(bool(enable))), "wr_set_timing")
