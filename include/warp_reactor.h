@@ -172,6 +172,11 @@ int wr_set_advance_kernel(wr_context *ctx, int kernel);
  * WR_TOO_MANY_STEPS failures, for testing the failure path. */
 int wr_set_integrator_max_steps(wr_context *ctx, int max_steps);
 
+/* wr_run enqueues its outer steps 8 at a time as one captured HIP graph
+ * (default 1), or as individual launches (0).  Runs with timing enabled use
+ * individual launches.  Results are identical. */
+int wr_set_graph(wr_context *ctx, int enable);
+
 /* Time the advance kernel launches of subsequent wr_run calls with HIP events
  * recorded on the context's stream (off by default). */
 int wr_set_timing(wr_context *ctx, int enable);
@@ -202,6 +207,17 @@ int wr_eval_integrate(wr_context *ctx, double *y, const double *tout, const doub
 /* ---- initial states, packed final states, comparison (host) ------------------ */
 /* make_collapse_state (reproducer.cpp:7533-7546) for n cells. */
 int wr_make_collapse_states(wr_collapse_state *cells, int num_cells);
+/* Benchmark-recipe initial states (no counterpart in the reference, which
+ * only has make_collapse_state): log-normal hydrogen density
+ * n_H = exp(sigma_ln_nh * z) cm^-3 with z standard normal (Box-Muller on
+ * SplitMix64 of seed ^ cell), temperature t_min, or log-uniform in
+ * [t_min, t_max] when t_max > t_min; ionisation 1e-4, H2 1e-6, He mass
+ * fraction 0.24, D/H 3e-5, electrons by charge balance, then normalised and
+ * e from the EOS as make_initial_state does (7533-7541).
+ *   16^3 recipe: seed 0, sigma 0.1, T = 300 K.
+ *   128^3 recipe: seed 1, sigma 1.0, T log-uniform in [100, 8000] K. */
+int wr_make_recipe_states(wr_collapse_state *cells, int num_cells, uint64_t seed,
+                          double sigma_ln_nh, double t_min, double t_max);
 /* make_packed_final_state (8289-8305). */
 int wr_pack_final_states(const wr_collapse_state *cells, int num_cells, int grid_dim,
                          wr_packed_final_state *out);

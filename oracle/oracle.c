@@ -471,6 +471,40 @@ void or_make_collapse_state(or_cell *c) {
     c->density_driver = c->current.rho;
 }
 
+/* Benchmark-recipe initial state of cell `index` (the build's own recipe; the
+ * reference has none): SplitMix64 chain h1 = sm(seed ^ index), h2 = sm(h1),
+ * h3 = sm(h2); Box-Muller z = sqrt(-2 ln u1) cos(2 pi u2), u1 in (0, 1];
+ * n_H = exp(sigma z); T = t_min, or t_min exp(u3 ln(t_max / t_min)) when
+ * t_max > t_min; H+ 1e-4, H2 1e-6, H the rest of n_H, He Y = 0.24, D/H 3e-5,
+ * other species 1e-40 (per n_H), electrons by charge balance; then rho,
+ * normalise and eos_rt as make_initial_state (reproducer.cpp:7533-7541). */
+uint64_t or_splitmix64(uint64_t v);
+void or_make_recipe_state(or_cell *c, int index, uint64_t seed, double sigma, double t_min,
+                          double t_max) {
+    const uint64_t h1 = or_splitmix64(seed ^ (uint64_t)index);
+    const uint64_t h2 = or_splitmix64(h1);
+    const uint64_t h3 = or_splitmix64(h2);
+    const double u1 = (double)((h1 >> 11) + 1) * 0x1p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1p-53;
+    const double u3 = (double)(h3 >> 11) * 0x1p-53;
+    const double z = sqrt(-2.0 * log(u1)) * cos((2.0 * k_pi) * u2);
+    const double nh = exp(sigma * z);
+    const double T = (t_max > t_min) ? t_min * exp(u3 * log(t_max / t_min)) : t_min;
+    memset(c, 0, sizeof(*c));
+    for (int n = 0; n < NSPEC; ++n) c->current.xn[n] = 1.0e-40 * nh;
+    c->current.xn[1] = 1.0e-4 * nh;
+    c->current.xn[8] = 1.0e-6 * nh;
+    c->current.xn[2] = nh - c->current.xn[1] - 2.0 * c->current.xn[8];
+    c->current.xn[5] = 3.0e-5 * nh;
+    c->current.xn[13] = (0.24 / (4.0 * (1.0 - 0.24))) * nh;
+    balance_charge(&c->current);
+    c->current.T = T;
+    c->current.rho = density_of(c->current.xn);
+    normalize(&c->current);
+    eos_rt(&c->current);
+    c->density_driver = c->current.rho;
+}
+
 /* splitmix64 (reproducer.cpp:7548-7553; Steele, Lea & Flood 2014). */
 uint64_t or_splitmix64(uint64_t v) {
     v += 0x9e3779b97f4a7c15ULL;

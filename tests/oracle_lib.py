@@ -44,6 +44,7 @@ class _Cpu:
         }
         if p == "or_":
             sig.update({"set_integrator_max_steps": (None, [_I]),
+                        "make_recipe_state": (None, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
                         "libm_exp": (_D, [_D]), "libm_log": (_D, [_D]), "libm_cbrt": (_D, [_D]),
                         "replay_cells": (None, [_VP, _VP, _I, _VP, _I, _I, _I, _D, _VP])})
         for name, (res, args) in sig.items():
@@ -137,6 +138,14 @@ class _Cpu:
         f = _I(1)
         r = self._advance_cell(_p(cell), completed, next_time, dt_grid, ctypes.byref(f))
         return r, f.value
+
+    def make_recipe_cells(self, n, dtype, seed, sigma, t_min, t_max):
+        cells = np.zeros(n, dtype=dtype)
+        item = cells.itemsize
+        base = cells.ctypes.data
+        for i in range(n):
+            self._make_recipe_state(_VP(base + i * item), i, seed, sigma, t_min, t_max)
+        return cells
 
     def run_gridwide(self, cells, perturb=True, max_steps=0):
         done = _I(0)

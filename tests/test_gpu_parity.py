@@ -256,3 +256,51 @@ def test_grid_against_reference_comparator(wr, oracle, tmp_path):
     summary = wr.compare_final_states(cells, g, path)
     print("comparator vs host-libm oracle:", summary)
     assert summary["pass"], summary["first_failure"]
+
+
+@pytest.mark.parametrize("recipe,grid,steps", [("uniform16", 8, 145), ("lognormal128", 4, 60)])
+def test_recipe_runs_bit_exact(wr, oracle_wr, recipe, grid, steps):
+    """Benchmark-recipe initial states (log-normal density, fixed or
+    log-uniform T) through the default advance kernel: final states and
+    counters identical to the oracle."""
+    n = grid ** 3
+    seed, sigma, t0, t1 = wr.RECIPES[recipe]
+    cells = wr.make_recipe_states(n, seed, sigma, t0, t1)
+    with wr.Context(n) as ctx:
+        ctx.upload(cells)
+        info = ctx.run(perturb=True, max_steps=steps)
+        ctx.download(cells)
+    ocells = oracle_wr.make_recipe_cells(n, wr.CELL_DTYPE, seed, sigma, t0, t1)
+    orr, ocompleted = oracle_wr.run_gridwide(ocells, True, max_steps=steps)
+    assert info["status"] == orr == 1
+    assert info["completed_global_steps"] == ocompleted
+    assert _cells_equal(cells, ocells)
+
+
+def test_graph_replays_match_eager_launches(wr):
+    """Outer steps enqueued as a captured HIP graph (8 steps per replay, tens
+    of replays per run) against individual launches: identical grids, dt
+    histories and run info, for two runs on one context (the graph is
+    reused), then for a smaller grid on the same context (captured again),
+    with eager work in between."""
+    n = 12 ** 3
+    init = wr.make_collapse_states(n)
+    results = {}
+    for graph in (False, True):
+        with wr.Context(n) as ctx:
+            ctx.set_graph(graph)
+            runs = []
+            for m, steps in ((n, 200), (n, 200), (n // 2, 77)):
+                cells = init[:m].copy()
+                ctx.upload(cells)
+                info = ctx.run(perturb=True, max_steps=steps)
+                ctx.download(cells)
+                runs.append((cells, info, ctx.dt_history()))
+                ctx.eval_rhs(np.full((4, 15), 1e-3))  # eager work between replays
+            results[graph] = runs
+    for (a, ia, da), (b, ib, db) in zip(results[False], results[True]):
+        assert ia == ib
+        assert np.array_equal(da, db)
+        assert _cells_equal(a, b)
+    # the repeated run on one context reproduces the first
+    assert _cells_equal(results[True][0][0], results[True][1][0])

@@ -19,7 +19,9 @@ from .runtime import (  # noqa: F401
     compare_final_states,
     library,
     library_path,
+    RECIPES,
     make_collapse_states,
+    make_recipe_states,
     pack_final_states,
     read_final_states,
     run_cells,

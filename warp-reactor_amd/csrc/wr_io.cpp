@@ -99,6 +99,57 @@ int wr_make_collapse_states(wr_collapse_state* cells, int num_cells) {
     return WR_SUCCESS;
 }
 
+// Synthetic initial states of the benchmark recipes (no counterpart in the
+// reference, whose only initial state is make_collapse_state).  Cell c draws
+// three SplitMix64 outputs h1 = splitmix64(seed ^ c), h2 = splitmix64(h1),
+// h3 = splitmix64(h2) and uniforms u = (h >> 11) 2^-53 (u1 shifted into
+// (0, 1]).  Box-Muller: z = sqrt(-2 ln u1) cos(2 pi u2); hydrogen density
+// n_H = exp(sigma z) cm^-3; T = t_min (t_max^... log-uniform when t_max >
+// t_min: T = t_min exp(u3 ln(t_max / t_min))).  Composition per n_H: H+ 1e-4,
+// H2 1e-6, H the rest of the hydrogen, He by mass fraction Y = 0.24
+// (n_He / n_H = Y / (4 (1 - Y))), D/H 3e-5, all other species 1e-40;
+// electrons by charge balance.  Then as make_initial_state
+// (reproducer.cpp:7533-7541): rho from the number densities, normalise,
+// e from eos_rt; the density driver starts at rho.
+int wr_make_recipe_states(wr_collapse_state* cells, int num_cells, uint64_t seed,
+                          double sigma_ln_nh, double t_min, double t_max) {
+    if (!cells || num_cells < 0 || !(sigma_ln_nh >= 0.0) || !std::isfinite(sigma_ln_nh) ||
+        !(t_min > 0.0) || !(t_max >= t_min) || !std::isfinite(t_max))
+        return WR_ERR_ARGUMENT;
+    const double two_pi = 2.0 * wr::kPi;
+    const double he_per_h = 0.24 / (4.0 * (1.0 - 0.24));
+    for (int c = 0; c < num_cells; ++c) {
+        const uint64_t h1 = wr::splitmix64(seed ^ static_cast<uint64_t>(c));
+        const uint64_t h2 = wr::splitmix64(h1);
+        const uint64_t h3 = wr::splitmix64(h2);
+        const double u1 = static_cast<double>((h1 >> 11U) + 1U) * 0x1p-53;
+        const double u2 = static_cast<double>(h2 >> 11U) * 0x1p-53;
+        const double u3 = static_cast<double>(h3 >> 11U) * 0x1p-53;
+        const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(two_pi * u2);
+        const double nh = std::exp(sigma_ln_nh * z);
+        const double T = (t_max > t_min) ? t_min * std::exp(u3 * std::log(t_max / t_min)) : t_min;
+        double xn[WR_NUM_SPEC];
+        for (int n = 0; n < WR_NUM_SPEC; ++n) xn[n] = 1.0e-40 * nh;
+        xn[1] = 1.0e-4 * nh;
+        xn[8] = 1.0e-6 * nh;
+        xn[2] = nh - xn[1] - 2.0 * xn[8];
+        xn[5] = 3.0e-5 * nh;
+        xn[13] = he_per_h * nh;
+        wr::balance_charge(xn);
+        wr_collapse_state s;
+        memset(&s, 0, sizeof(s));
+        const double rho = wr::density_of(xn);
+        wr::normalize(xn, rho);
+        s.current.rho = rho;
+        s.current.T = T;
+        s.current.e = wr::eos_energy(xn, T);
+        memcpy(s.current.xn, xn, sizeof(xn));
+        s.density_driver = rho;
+        cells[c] = s;
+    }
+    return WR_SUCCESS;
+}
+
 // make_packed_final_state (reproducer.cpp:8289-8305) for the cell at `index`.
 static wr_packed_final_state pack_one(const wr_collapse_state& s, int index, int grid_dim) {
     wr_packed_final_state p;

@@ -812,6 +812,10 @@ struct wr_context {
     // HIP-event timing of the advance launches of wr_run (when enabled)
     static constexpr int kChunk = 8;
     bool timing = false;
+    // one chunk of outer steps captured as a HIP graph (used when not timing)
+    bool use_graph = true;
+    hipGraphExec_t chunk_exec = nullptr;
+    int graph_n = -1, graph_kernel = -1;  // what the captured launches were built for
     hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
     double adv_ms = 0.0;
     int adv_launches = 0;
@@ -836,6 +840,7 @@ void wr_context_destroy(wr_context* ctx) {
     (void)hipFree(ctx->dt_hist);
     (void)hipFree(ctx->staging);
     if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
+    if (ctx->chunk_exec) (void)hipGraphExecDestroy(ctx->chunk_exec);
     for (int k = 0; k < wr_context::kChunk; ++k) {
         if (ctx->adv_start[k]) (void)hipEventDestroy(ctx->adv_start[k]);
         if (ctx->adv_stop[k]) (void)hipEventDestroy(ctx->adv_stop[k]);
@@ -1038,6 +1043,51 @@ static void fill_info(wr_context* ctx) {
     ctx->hist_len = len < 0 ? 0 : (len > wr_context::kHistCap ? wr_context::kHistCap : len);
 }
 
+// Enqueue `todo` outer steps: prepare, plan, (order,) advance, commit each.
+static int enqueue_steps(wr_context* ctx, int todo, bool timed) {
+    int st;
+    for (int k = 0; k < todo; ++k) {
+        if ((st = launch_prepare(ctx)) != WR_SUCCESS) return st;
+        if (timed) WR_HIP(hipEventRecord(ctx->adv_start[k], ctx->stream), "hipEventRecord");
+        if ((st = launch_advance(ctx)) != WR_SUCCESS) return st;
+        if (timed) WR_HIP(hipEventRecord(ctx->adv_stop[k], ctx->stream), "hipEventRecord");
+        hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
+        WR_HIP(hipGetLastError(), "commit launch");
+    }
+    return WR_SUCCESS;
+}
+
+// A full chunk of outer steps as one HIP graph: the kernels' arguments are
+// fixed for a context (the control block carries everything that changes),
+// except the cell count and the advance kernel, so the graph is captured
+// again only when those change.  The graph holds kernel nodes only.
+static int chunk_graph(wr_context* ctx) {
+    if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == ctx->advance_kernel)
+        return WR_SUCCESS;
+    if (ctx->chunk_exec) {
+        WR_HIP(hipGraphExecDestroy(ctx->chunk_exec), "hipGraphExecDestroy");
+        ctx->chunk_exec = nullptr;
+    }
+    int st = queue_grid(ctx);  // occupancy queries stay outside the capture
+    if (st != WR_SUCCESS) return st;
+    hipGraph_t g = nullptr;
+    WR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+           "hipStreamBeginCapture");
+    st = enqueue_steps(ctx, wr_context::kChunk, false);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (st != WR_SUCCESS) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    WR_HIP(e, "hipStreamEndCapture");
+    const hipError_t ie = hipGraphInstantiate(&ctx->chunk_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    WR_HIP(ie, "hipGraphInstantiate");
+    ctx->graph_n = ctx->n;
+    ctx->graph_kernel = ctx->advance_kernel;
+    return WR_SUCCESS;
+}
+
 int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_steps,
            double grid_time) {
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_run: null context");
@@ -1057,13 +1107,12 @@ int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_ste
     for (int issued = 0; issued < max_steps;) {
         const int todo = (max_steps - issued < chunk) ? max_steps - issued : chunk;
         const int steps_before = ctx->ctl_host->step;
-        for (int k = 0; k < todo; ++k) {
-            if ((st = launch_prepare(ctx)) != WR_SUCCESS) return st;
-            if (ctx->timing) WR_HIP(hipEventRecord(ctx->adv_start[k], ctx->stream), "hipEventRecord");
-            if ((st = launch_advance(ctx)) != WR_SUCCESS) return st;
-            if (ctx->timing) WR_HIP(hipEventRecord(ctx->adv_stop[k], ctx->stream), "hipEventRecord");
-            hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
-            WR_HIP(hipGetLastError(), "commit launch");
+        if (ctx->use_graph && !ctx->timing) {
+            // steps past max_steps in the last chunk return at once (step_limit)
+            if ((st = chunk_graph(ctx)) != WR_SUCCESS) return st;
+            WR_HIP(hipGraphLaunch(ctx->chunk_exec, ctx->stream), "hipGraphLaunch");
+        } else if ((st = enqueue_steps(ctx, todo, ctx->timing)) != WR_SUCCESS) {
+            return st;
         }
         issued += todo;
         if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
@@ -1112,6 +1161,12 @@ int wr_set_advance_kernel(wr_context* ctx, int kernel) {
 int wr_set_integrator_max_steps(wr_context* ctx, int max_steps) {
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_set_integrator_max_steps: null context");
     ctx->integ_max_steps = max_steps > 0 ? max_steps : kMaxSteps;
+    return WR_SUCCESS;
+}
+
+int wr_set_graph(wr_context* ctx, int enable) {
+    if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_set_graph: null context");
+    ctx->use_graph = enable != 0;
     return WR_SUCCESS;
 }
 

@@ -81,6 +81,7 @@ _SIGS = {
     "wr_set_advance_kernel": (_I, [_VP, _I]),
     "wr_set_timing": (_I, [_VP, _I]),
     "wr_set_integrator_max_steps": (_I, [_VP, _I]),
+    "wr_set_graph": (_I, [_VP, _I]),
     "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_eval_rhs": (_I, [_VP, _VP, _I, _VP]),
     "wr_eval_math": (_I, [_VP, _I, _VP, _I, _VP]),
@@ -89,6 +90,7 @@ _SIGS = {
     "wr_eval_lu_solve": (_I, [_VP, _VP, _VP, _I, _VP]),
     "wr_eval_integrate": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
     "wr_make_collapse_states": (_I, [_VP, _I]),
+    "wr_make_recipe_states": (_I, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
     "wr_pack_final_states": (_I, [_VP, _I, _I, _VP]),
     "wr_write_final_states": (_I, [_VP, _I, _I, ctypes.c_char_p]),
     "wr_read_final_states": (_I, [ctypes.c_char_p, _I, _VP]),
@@ -128,6 +130,22 @@ def _check(code, what):
 def make_collapse_states(n):
     cells = np.zeros(n, dtype=CELL_DTYPE)
     _check(library().wr_make_collapse_states(_ptr(cells), n), "wr_make_collapse_states")
+    return cells
+
+
+RECIPES = {
+    # name: (seed, sigma of ln n_H, T_min, T_max)
+    "uniform16": (0, 0.1, 300.0, 300.0),
+    "lognormal128": (1, 1.0, 100.0, 8000.0),
+}
+
+
+def make_recipe_states(n, seed=0, sigma_ln_nh=0.1, t_min=300.0, t_max=None):
+    """Benchmark-recipe initial states (wr_make_recipe_states)."""
+    cells = np.zeros(n, dtype=CELL_DTYPE)
+    _check(library().wr_make_recipe_states(_ptr(cells), n, seed, sigma_ln_nh, t_min,
+                                           t_min if t_max is None else t_max),
+           "wr_make_recipe_states")
     return cells
 
 
@@ -229,6 +247,10 @@ class Context:
         """Integrator step limit of subsequent runs (<= 0: 1e7, the reference's)."""
         _check(self._lib.wr_set_integrator_max_steps(self._h, int(max_steps)),
                "wr_set_integrator_max_steps")
+
+    def set_graph(self, enable=True):
+        """Enqueue outer steps as captured HIP graphs (default) or launches."""
+        _check(self._lib.wr_set_graph(self._h, int(bool(enable))), "wr_set_graph")
 
     def set_timing(self, enable=True):
         _check(self._lib.wr_set_timing(self._h, int(bool(enable))), "wr_set_timing")
