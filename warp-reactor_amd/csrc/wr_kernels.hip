@@ -73,6 +73,7 @@ struct Ctl {
     int hist_cap;       // capacity of dt_hist
     double* dt_hist;    // dt_grid of every outer step of the run (device)
     int queue;          // next queue position to hand out in the queue-fed advance
+    int queue_solo;     // next of the positions reserved for the solo wave
     int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
     int bin_count[16];  // cells per attempt-count bin, for the queue order
     int bin_used[16];   // queue positions already handed to blocks, per bin
@@ -258,6 +259,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(Ctl* ctl, const double* par
     ctl->limiting_cell = cell;
     ctl->integrated = 0;
     ctl->queue = 0;
+    ctl->queue_solo = 0;
     const int k = ctl->step - ctl->step0;
     if (ctl->dt_hist && k >= 0 && k < ctl->hist_cap) ctl->dt_hist[k] = dt;
 }
@@ -407,9 +409,18 @@ constexpr int kMaxSteps = 10000000;  // configure_ros2s (reproducer.cpp:7589)
 // resolve, which keeps the array in memory instead of registers.
 constexpr int kPrivLen = 2 * kNeqs * kNeqs + kVecLen + kStageSlots;
 
+// The first kSoloLanes queue positions -- the cells that took the most
+// attempts last time -- go to the first wave of block 0, which runs alone on
+// its CU (the block's other waves leave at once): the one cell that needs
+// tens of attempts then does not share instruction issue and fetch with
+// three other waves while the bulk of the cells is being advanced.
+constexpr int kSoloLanes = 64;
+
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero) {
     using C = Ros2sCoef;
     if (ctl->done) return;
+    bool solo = blockIdx.x == 0;
+    if (solo && threadIdx.x >= kSoloLanes) return;
     double priv[kPrivLen];
     double* const p = priv + zero;
     ThreadWs w;
@@ -432,7 +443,16 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     for (;;) {
         if (c < 0) {
             // ---- take the next cell: advance_collapse_gridwide_kernel's preamble
-            const int q = atomicAdd(&ctl->queue, 1);
+            int q;
+            if (solo) {
+                q = atomicAdd(&ctl->queue_solo, 1);
+                if (q >= kSoloLanes) {
+                    solo = false;  // reserved positions used up: join the common queue
+                    continue;
+                }
+            } else {
+                q = kSoloLanes + atomicAdd(&ctl->queue, 1);
+            }
             if (q >= s.n || ctl->failure != kSuccess) break;
             c = s.order[q];
             if (s.completed[c] != completed) {
