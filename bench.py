@@ -116,6 +116,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--advance-kernel", type=int, default=1,
                     help="1: queue-fed persistent kernel (default); 0: one thread per cell")
+    ap.add_argument("--recipe", default="reference",
+                    choices=["reference", "uniform16", "lognormal128"],
+                    help="initial states: the reference's make_collapse_state (default) or a "
+                         "benchmark recipe (log-normal density, fixed / log-uniform T)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the thread-per-cell comparison run")
     args = ap.parse_args()
@@ -126,7 +130,15 @@ def main():
 
     wr = importlib.import_module("warp-reactor_amd")
     n = args.grid ** 3
-    init = wr.make_collapse_states(n)
+    if args.recipe == "reference":
+        init = wr.make_collapse_states(n)
+        data = ("synthetic: reference initial state (make_collapse_state) with splitmix64 "
+                "density perturbations")
+    else:
+        seed, sigma, t0, t1 = wr.RECIPES[args.recipe]
+        init = wr.make_recipe_states(n, seed, sigma, t0, t1)
+        data = (f"synthetic: recipe {args.recipe} (n_H = exp({sigma} z) cm^-3, seed {seed}, "
+                f"T {t0:g}-{t1:g} K) with splitmix64 density perturbations")
     ctx = wr.Context(n)
     ctx.set_advance_kernel(args.advance_kernel)
     # prime: load the kernels with one outer step (not timed, not counted)
@@ -170,7 +182,7 @@ def main():
     achieved = per_launch_flops / mean_launch_s / 1e12 if adv_launches else None
     value = total_adv / total_s
     line = {
-        "metric": "cell-advances/s, 64^3 gridwide-independent ROS2S collapse (plain FP64)",
+        "metric": f"cell-advances/s, {args.grid}^3 gridwide-independent ROS2S collapse (plain FP64)",
         "value": value,
         "unit": "cell-advances/s",
         "n_gpus": 1,
@@ -181,9 +193,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: reference initial state (make_collapse_state) with splitmix64 "
-                "density perturbations",
-        "config": {"workload": f"primordial collapse {args.grid}^3 = {n} cells, <= "
+        "data": data,
+        "config": {"workload": f"primordial collapse {args.grid}^3 = {n} cells, "
+                               f"{args.recipe} initial states, <= "
                                f"{args.max_outer} outer steps, perturb every 20, plain FP64",
                    "cells": n, "outer_steps": infos[-1]["completed_global_steps"],
                    "cell_advances_per_run": infos[-1]["cell_advances"],
@@ -199,7 +211,7 @@ def main():
         "counters": counters,
     }
     kname = line["roofline"]["kernel"]
-    traffic = measured_traffic(kname, args.grid)
+    traffic = measured_traffic(kname, args.grid) if args.recipe == "reference" else None
     line["roofline"]["traffic"] = traffic
     if traffic and adv_launches:
         line["roofline"]["traffic_gbs"] = traffic / mean_launch_s / 1e9

@@ -1,0 +1,100 @@
+"""The reproducer-compatible CLI on the GPU: stdout lines of
+reproducer.cpp:8611-8703 for the gridwide-independent device policy, the
+packed final-state file (8357-8378) and the comparison flow (8416-8578).
+
+The run is bit-identical to the oracle built with the device libm, so every
+printed summary number and every byte of the written file is checked
+against the oracle's grid.  The golden file tests/golden/final_states_grid2.bin
+was written by the reference's own writer from its CPU arithmetic; the
+device run must pass the reference comparator against it."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(REPO, "warp-reactor_amd", "bin", "wr_reproducer")
+GOLD2 = os.path.join(REPO, "tests", "golden", "final_states_grid2.bin")
+
+
+def run(args, cwd):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=300, cwd=cwd)
+
+
+def _summary(lines, label):
+    line = next(l for l in lines if l.startswith(label + ": "))
+    return [float(x) for x in re.findall(r"[-+0-9.eE]+", line[len(label) + 2:])]
+
+
+def _med(v):
+    v = np.sort(np.asarray(v, dtype=np.float64))
+    n = len(v)
+    return v[n // 2] if n % 2 else (v[n // 2 - 1] + v[n // 2]) / 2
+
+
+@pytest.fixture(scope="module")
+def oracle_grid2(wr, oracle_wr):
+    cells = oracle_wr.make_cells(8, wr.CELL_DTYPE)
+    r, completed = oracle_wr.run_gridwide(cells, True)
+    assert r == 1
+    return cells, completed
+
+
+def test_cli_grid2_stdout_file_and_pass(tmp_path, wr, oracle_grid2):
+    ocells, ocompleted = oracle_grid2
+    r = run(["--grid", "2", "--compare-final-state", GOLD2], tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[:5] == ["Primordial chemistry collapse grid with ROS2S", "grid: 2^3 (8 cells)",
+                         "perturbations: enabled", "driver policy: gridwide-independent",
+                         f"completed global collapse steps: {ocompleted}"]
+    st = ocells["stats"]
+    for label, v in (("cell completed steps", ocells["completed_steps"]),
+                     ("cell physical time", ocells["time"]),
+                     ("cell density driver", ocells["density_driver"]),
+                     ("ROS2S internal steps", st["internal_steps"]),
+                     ("ROS2S rhs calls", st["rhs_calls"]),
+                     ("ROS2S jacobian calls", st["jacobian_calls"]),
+                     ("ROS2S decompositions", st["decompositions"]),
+                     ("ROS2S linear solves", st["linear_solves"])):
+        assert _summary(lines, label) == [float(np.min(v)), float(_med(v)), float(np.max(v))], label
+    acc = _summary(lines, "ROS2S accepted/rejected")
+    a, j = st["accepted_steps"], st["rejected_steps"]
+    assert acc == [a.min(), j.min(), _med(a), _med(j), a.max(), j.max()]
+    assert any(l.startswith("wall time: ") and l.endswith(" s") for l in lines)
+    assert f"final-state comparison file: {GOLD2}" in lines
+    assert any(l.startswith("final-state comparison: PASS (max deuterium-bearing") for l in lines)
+    assert "final states: final_states_grid2_hip.bin (8 packed records, 172 bytes each)" in lines
+    assert lines[-1] == "PASS" and lines[-2] == ""
+    data = (tmp_path / "final_states_grid2_hip.bin").read_bytes()
+    assert data == wr.pack_final_states(ocells, 2).tobytes()
+
+
+def test_cli_rotates_and_reports_failures(tmp_path, wr, oracle_grid2):
+    """A second run moves the existing file to .old.NNNNNN, NNNNNN a random
+    six-digit suffix (8307-8355); a
+    comparison against a file with one perturbed temperature fails, prints
+    FAIL, names the field and writes the failures CSV (8100-8142)."""
+    ocells, _ = oracle_grid2
+    assert run(["--grid", "2", "--no-compare-final-state"], tmp_path).returncode == 0
+    bad = wr.pack_final_states(ocells, 2)
+    bad["T"][3] *= 1.01
+    bad_path = tmp_path / "bad.bin"
+    bad_path.write_bytes(bad.tobytes())
+    r = run(["--grid", "2", f"--compare-final-state={bad_path}"], tmp_path)
+    assert r.returncode == 1
+    lines = r.stdout.splitlines()
+    assert lines[-1] == "FAIL"
+    assert any(l.startswith("final-state comparison: FAIL") for l in lines)
+    assert any(l.startswith("first comparison failure:") and "T" in l for l in lines)
+    old = [p.name for p in tmp_path.iterdir() if p.name.startswith("final_states_grid2_hip.bin.old.")]
+    assert len(old) == 1 and re.fullmatch(r"final_states_grid2_hip\.bin\.old\.[0-9]{6}", old[0])
+    assert any(l == f"existing final states moved to: final_states_grid2_hip.bin.old.{old[0][-6:]}"
+               for l in lines)
+    assert (tmp_path / "final_states_grid2_hip.bin").exists()
+    csv = (tmp_path / "final_state_comparison_failures_hip.csv").read_text().splitlines()
+    assert len(csv) >= 2 and "cell" in csv[0]
