@@ -167,6 +167,24 @@ int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_tim
  * network code); 0 = one thread per cell.  Results are identical. */
 int wr_set_advance_kernel(wr_context *ctx, int kernel);
 
+/* Factorisation of I - gamma h J in the queue-fed kernel of subsequent steps
+ * (no counterpart in the reference, which factorises in FP64 only:
+ * reproducer.cpp:7159-7171).  mode 0 (default): FP64, results identical to the
+ * reference arithmetic.  mode 1: E = fac I - J rounded to FP32 and factorised
+ * in FP32; every stage solve takes `sweeps` (1..4) refinement sweeps with the
+ * residual b - E x in FP64; the RHS, energy equation, error norm and step
+ * controller stay FP64.  The unknowns are equilibrated by exact powers of two,
+ * 2^ilogb(atol_i + rtol_i |y_i|), before rounding to FP32.  FP64 factors are
+ * used for the rest of an attempt when the FP32 factors are singular or a
+ * refinement does not converge (last correction > 1e-6 (atol_i + rtol_i |y_i|)
+ * in some component).  Mode 1 needs advance kernel 1 (runs otherwise fail
+ * with WR_ERR_ARGUMENT). */
+int wr_set_lu_precision(wr_context *ctx, int mode, int sweeps);
+/* Fallback counts of the last wr_run in mode 1: attempts whose FP32 factors
+ * were singular, stage solves whose refinement did not converge. */
+int wr_mixed_fallbacks(const wr_context *ctx, unsigned long long *lu_fallbacks,
+                       unsigned long long *refine_fallbacks);
+
 /* Integrator step limit of the burns of subsequent runs (<= 0: 1e7, the
  * configure_ros2s value, reproducer.cpp:7589).  Lower values force
  * WR_TOO_MANY_STEPS failures, for testing the failure path. */
@@ -196,6 +214,13 @@ int wr_eval_lu(wr_context *ctx, double *a /* n x 225, in/out */, int n, int32_t 
                int32_t *info /* n */);
 int wr_eval_lu_solve(wr_context *ctx, const double *lu, const int32_t *ipvt, int n,
                      double *x /* n x 15, in/out */);
+/* One stage solve of the FP32-factorisation form (wr_set_lu_precision mode 1)
+ * per row: E = fac I - J equilibrated by 2^ilogb(atol_i + rtol_i |y_i|) and
+ * factorised in FP32 (factors to lu32, n x 225 floats), then x (in: b) solved
+ * with `sweeps` (0..4) FP64-residual refinement sweeps.  conv[i]: -1 singular
+ * FP32 factors, 1 converged, 0 not converged. */
+int wr_eval_solve_mixed(wr_context *ctx, const double *jac, const double *fac, const double *y,
+                        int n, int sweeps, double *x, int32_t *conv, float *lu32);
 /* RODAS<PrimordialChem>::integrate from t = 0 to tout[i] with first step
  * dt0[i] and the collapse tolerances; counters n x 7 = n_step, n_rhs, n_jac,
  * n_decomp, n_solve, n_accept, n_reject; t_end/dt_end as the integrator

@@ -259,6 +259,91 @@ void or_lu_solve(const double *lu, const int *ipvt, double *x) {
     }
 }
 
+/* ---- FP32 factorisation with FP64 refinement (no reference counterpart) --
+ * The same LINPACK elimination in single precision on E rounded to float. */
+int or_lu32(float *a, int *ipvt) {
+    int info = 0;
+    for (int k = 0; k < NEQ - 1; ++k) {
+        int p = k;
+        float best = fabsf(a[k * NEQ + k]);
+        for (int i = k + 1; i < NEQ; ++i) {
+            const float v = fabsf(a[i * NEQ + k]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        ipvt[k] = p;
+        if (a[p * NEQ + k] == 0.0f) {
+            info = k + 1;
+            continue;
+        }
+        if (p != k) {
+            for (int j = k; j < NEQ; ++j) {
+                const float t = a[p * NEQ + j];
+                a[p * NEQ + j] = a[k * NEQ + j];
+                a[k * NEQ + j] = t;
+            }
+        }
+        const float m = -1.0f / a[k * NEQ + k];
+        for (int i = k + 1; i < NEQ; ++i) a[i * NEQ + k] *= m;
+        for (int j = k + 1; j < NEQ; ++j) {
+            const float t = a[k * NEQ + j];
+            for (int i = k + 1; i < NEQ; ++i) a[i * NEQ + j] += t * a[i * NEQ + k];
+        }
+    }
+    ipvt[NEQ - 1] = NEQ - 1;
+    if (a[(NEQ - 1) * NEQ + NEQ - 1] == 0.0f) info = NEQ;
+    return info;
+}
+
+void or_lu_solve32(const float *lu, const int *ipvt, float *x) {
+    for (int k = 0; k < NEQ - 1; ++k) {
+        const int p = ipvt[k];
+        const float t = x[p];
+        if (p != k) {
+            x[p] = x[k];
+            x[k] = t;
+        }
+        for (int j = k + 1; j < NEQ; ++j) x[j] += t * lu[j * NEQ + k];
+    }
+    for (int k = NEQ - 1; k >= 0; --k) {
+        x[k] /= lu[k * NEQ + k];
+        const float t = -x[k];
+        for (int j = 0; j < k; ++j) x[j] += t * lu[j * NEQ + k];
+    }
+}
+
+static int g_lu_mode = 0, g_lu_sweeps = 2;
+static uint64_t g_lu_fallbacks = 0, g_refine_fallbacks = 0;
+
+/* mode 0: FP64 factors (the reference); mode 1: FP32 factors, `sweeps`
+ * refinement sweeps with FP64 residuals per stage solve; the unknowns are
+ * equilibrated by the powers of two 2^ilogb(atol_i + rtol_i |y_i|). */
+void or_set_lu_precision(int mode, int sweeps) {
+    g_lu_mode = mode;
+    g_lu_sweeps = sweeps;
+}
+void or_mixed_fallbacks(uint64_t *lu_fb, uint64_t *refine_fb) {
+    *lu_fb = g_lu_fallbacks;
+    *refine_fb = g_refine_fallbacks;
+}
+void or_reset_mixed_fallbacks(void) {
+    g_lu_fallbacks = 0;
+    g_refine_fallbacks = 0;
+}
+
+#define OR_REFINE_TOL 1.0e-6
+
+/* Stage solve of the FP32-factorisation form: x (in: b) from the FP32
+ * factors, then sweeps of x += E32^-1 (b - E x) with E = fac I - J rebuilt in
+ * FP64 (E_ij = -J_ij, plus fac on the diagonal) and the row sums taken in
+ * ascending column order; FP32 quantities are scaled by 2^-ks_i (unknown i)
+ * before rounding.  Converged when the last correction is at most
+ * 1e-6 (atol_i + rtol_i |y_i|) in every component. */
+static int solve_mixed(const float *lu32, const int *ip32, const double *fjac, double fac,
+                       const double *y, int sweeps, double *x);
+
 /* ---- ROS2S (reproducer.cpp:7066-7347) ---------------------------------- */
 static const double c_gamma = 0.292893218813452, c_ct2 = 0.585786437626905,
                     c_a21 = 2.0000000000000036, c_a31 = 6.828427124746214,
@@ -277,6 +362,113 @@ typedef struct {
     int n_step, n_rhs, n_jac, n_decomp, n_solve, n_accept, n_reject;
 } or_counters;
 
+static void mixed_scales(const double *y, int *ks) {
+    for (int i = 0; i < NEQ; ++i) ks[i] = ilogb(atol_of(i) + rtol_of(i) * fabs(y[i]));
+}
+
+static int solve_mixed(const float *lu32, const int *ip32, const double *fjac, double fac,
+                       const double *y, int sweeps, double *x) {
+    double b[NEQ];
+    float xf[NEQ];
+    int ks[NEQ];
+    mixed_scales(y, ks);
+    for (int i = 0; i < NEQ; ++i) {
+        b[i] = x[i];
+        xf[i] = (float)ldexp(x[i], -ks[i]);
+    }
+    or_lu_solve32(lu32, ip32, xf);
+    for (int i = 0; i < NEQ; ++i) x[i] = ldexp((double)xf[i], ks[i]);
+    int conv = 1;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        for (int i = 0; i < NEQ; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < NEQ; ++j) {
+                const double v = -fjac[i * NEQ + j];
+                const double eij = (i == j) ? v + fac : v;
+                acc += eij * x[j];
+            }
+            xf[i] = (float)ldexp(b[i] - acc, -ks[i]);
+        }
+        or_lu_solve32(lu32, ip32, xf);
+        conv = 1;
+        for (int i = 0; i < NEQ; ++i) {
+            const double d = ldexp((double)xf[i], ks[i]);
+            x[i] += d;
+            const double sk = atol_of(i) + rtol_of(i) * fabs(y[i]);
+            conv = conv && (fabs(d) <= OR_REFINE_TOL * sk);
+        }
+    }
+    return conv;
+}
+
+/* One stage solve of the FP32-factorisation form on its own (tests): E = fac I
+ * - J equilibrated and factorised in FP32, then x (in: b) solved with `sweeps`
+ * refinement sweeps.  Returns -1 if the FP32 factors are singular, else
+ * whether the refinement converged. */
+/* The FP32 factors of the equilibrated E = fac I - J (tests). */
+int or_mixed_lu(const double *fjac, double fac, const double *y, float *e32, int *ip32) {
+    int ks[NEQ];
+    mixed_scales(y, ks);
+    for (int i = 0; i < NEQ; ++i)
+        for (int j = 0; j < NEQ; ++j) {
+            const double v = -fjac[i * NEQ + j];
+            e32[i * NEQ + j] = (float)ldexp((i == j) ? v + fac : v, ks[j] - ks[i]);
+        }
+    return or_lu32(e32, ip32);
+}
+
+int or_solve_mixed(const double *fjac, double fac, const double *y, int sweeps, double *x) {
+    float e32[NEQ * NEQ];
+    int ip32[NEQ], ks[NEQ];
+    mixed_scales(y, ks);
+    for (int i = 0; i < NEQ; ++i)
+        for (int j = 0; j < NEQ; ++j) {
+            const double v = -fjac[i * NEQ + j];
+            e32[i * NEQ + j] = (float)ldexp((i == j) ? v + fac : v, ks[j] - ks[i]);
+        }
+    if (or_lu32(e32, ip32) != 0) return -1;
+    return solve_mixed(e32, ip32, fjac, fac, y, sweeps, x);
+}
+
+/* Hooks for the adjoint oracle (adjoint.c, built into the OR_ADJOINT
+ * library): with rate multipliers set, the RHS and Jacobian come from the
+ * generated f_theta code (identical bits when every multiplier is 1), and the
+ * step sizes of accepted attempts are recorded.  Per thread. */
+typedef void (*or_fj_fn)(const double *y, const double *M, double *f, double *J);
+typedef void (*or_f_fn)(const double *y, const double *M, double *f);
+static _Thread_local const double *t_M = NULL;
+static _Thread_local or_fj_fn t_fj = NULL;
+static _Thread_local or_f_fn t_f = NULL;
+static _Thread_local double *t_hrec = NULL;
+static _Thread_local int t_nh = 0, t_hcap = 0;
+
+void or_set_theta_hooks(const double *M, or_fj_fn fj, or_f_fn f) {
+    t_M = M;
+    t_fj = fj;
+    t_f = f;
+}
+void or_set_h_recorder(double *hrec, int hcap) {
+    t_hrec = hrec;
+    t_hcap = hcap;
+    t_nh = 0;
+}
+int or_h_recorded(void) { return t_nh; }
+
+static void hook_jac(const double *y, double *jac) {
+    if (t_M) {
+        double f[NEQ];
+        t_fj(y, t_M, f, jac);
+    } else {
+        or_jac(y, jac);
+    }
+}
+static void hook_rhs(const double *y, double *f) {
+    if (t_M)
+        t_f(y, t_M, f);
+    else
+        or_rhs(y, f);
+}
+
 /* RODAS<PrimordialChem>::integrate from t=0 to tout; y updated in place.
  * The checks on safe/fac/tolerances (7208-7218) cannot fail for the
  * collapse configuration and are kept for completeness. */
@@ -286,6 +478,8 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
     double fjac[NEQ * NEQ], e[NEQ * NEQ];
     double ynew[NEQ], ak1[NEQ], ak2[NEQ], work[NEQ], dy[NEQ];
     int ip[NEQ];
+    float e32[NEQ * NEQ];
+    int ip32[NEQ];
     double t = 0.0;
     int result;
     if (max_steps <= 0) max_steps = 10000000;
@@ -326,7 +520,7 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                 h = tout - x;
                 last = 1;
             }
-            or_jac(y, fjac);
+            hook_jac(y, fjac);
             c.n_jac += 1;
             for (;;) {
                 const double fac = 1.0 / (h * c_gamma);
@@ -334,7 +528,40 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                     for (int j = 0; j < NEQ; ++j) e[i * NEQ + j] = -fjac[i * NEQ + j];
                     e[i * NEQ + i] += fac;
                 }
-                if (or_lu(e, ip) != 0) {
+                int use64 = (g_lu_mode == 0), info;
+                if (!use64) {
+                    int ks[NEQ];
+                    mixed_scales(y, ks);
+                    for (int i = 0; i < NEQ; ++i)
+                        for (int j = 0; j < NEQ; ++j)
+                            e32[i * NEQ + j] = (float)ldexp(e[i * NEQ + j], ks[j] - ks[i]);
+                    info = or_lu32(e32, ip32);
+                    if (info != 0) { /* singular in FP32: FP64 factors for this attempt */
+                        __atomic_fetch_add(&g_lu_fallbacks, 1, __ATOMIC_RELAXED);
+                        info = or_lu(e, ip);
+                        use64 = 1;
+                    }
+                } else {
+                    info = or_lu(e, ip);
+                }
+#define OR_STAGE_SOLVE(v)                                                              \
+    do {                                                                               \
+        if (!use64) {                                                                  \
+            double b_[NEQ];                                                            \
+            memcpy(b_, (v), sizeof(b_));                                               \
+            if (solve_mixed(e32, ip32, fjac, fac, y, g_lu_sweeps, (v))) break;         \
+            __atomic_fetch_add(&g_refine_fallbacks, 1, __ATOMIC_RELAXED);             \
+            for (int i_ = 0; i_ < NEQ; ++i_) {                                         \
+                for (int j_ = 0; j_ < NEQ; ++j_) e[i_ * NEQ + j_] = -fjac[i_ * NEQ + j_];  \
+                e[i_ * NEQ + i_] += fac;                                               \
+            }                                                                          \
+            if (or_lu(e, ip) != 0) break; /* keep the refined FP32 result */           \
+            use64 = 1;                                                                 \
+            memcpy((v), b_, sizeof(b_));                                               \
+        }                                                                              \
+        or_lu_solve(e, ip, (v));                                                       \
+    } while (0)
+                if (info != 0) {
                     nsing += 1;
                     if (nsing >= 5) {
                         *t_end = x; *dt_end = h; result = OR_LU_ERROR; goto done;
@@ -346,24 +573,24 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                 }
                 c.n_decomp += 1;
 
-                or_rhs(y, ak1);
-                or_lu_solve(e, ip, ak1);
+                hook_rhs(y, ak1);
+                OR_STAGE_SOLVE(ak1);
                 c.n_solve += 1;
                 for (int i = 0; i < NEQ; ++i) {
                     ynew[i] = y[i] + c_a21 * ak1[i];
                     ak2[i] = (c_c21 / h) * ak1[i];
                 }
-                or_rhs(ynew, dy);
+                hook_rhs(ynew, dy);
                 for (int i = 0; i < NEQ; ++i) ak2[i] += dy[i];
-                or_lu_solve(e, ip, ak2);
+                OR_STAGE_SOLVE(ak2);
                 c.n_solve += 1;
                 for (int i = 0; i < NEQ; ++i) {
                     ynew[i] = y[i] + c_a31 * ak1[i] + c_a32 * ak2[i];
                     work[i] = (c_c31 * ak1[i] + c_c32 * ak2[i]) / h;
                 }
-                or_rhs(ynew, dy);
+                hook_rhs(ynew, dy);
                 for (int i = 0; i < NEQ; ++i) work[i] += dy[i];
-                or_lu_solve(e, ip, work);
+                OR_STAGE_SOLVE(work);
                 c.n_solve += 1;
                 for (int i = 0; i < NEQ; ++i) {
                     const double ak3 = work[i];
@@ -398,6 +625,10 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                     }
                     hacc = h;
                     erracc = ref_max(1.0e-2, err);
+                    if (t_hrec) {
+                        if (t_nh < t_hcap) t_hrec[t_nh] = h;
+                        t_nh += 1;
+                    }
                     memcpy(y, ynew, sizeof(ynew));
                     x += h;
                     if (fabs(hnew) > hmaxn) hnew = hmaxn;
@@ -727,3 +958,7 @@ void or_rhs_staged(const double *y, double *dydt) {
     double stg[CHEM_RHS_NSLOT];
     chem_rhs_staged(T, X, stg, (double *)0, dydt);
 }
+
+#ifdef OR_ADJOINT
+#include "adjoint.c"
+#endif

@@ -44,6 +44,12 @@ class _Cpu:
         }
         if p == "or_":
             sig.update({"set_integrator_max_steps": (None, [_I]),
+                        "set_lu_precision": (None, [_I, _I]),
+                        "mixed_fallbacks": (None, [_VP, _VP]),
+                        "reset_mixed_fallbacks": (None, []),
+                        "lu32": (_I, [_VP, _VP]), "lu_solve32": (None, [_VP, _VP, _VP]),
+                        "solve_mixed": (_I, [_VP, _D, _VP, _I, _VP]),
+                        "mixed_lu": (_I, [_VP, _D, _VP, _VP, _VP]),
                         "make_recipe_state": (None, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
                         "libm_exp": (_D, [_D]), "libm_log": (_D, [_D]), "libm_cbrt": (_D, [_D]),
                         "replay_cells": (None, [_VP, _VP, _I, _VP, _I, _I, _I, _D, _VP])})
@@ -147,6 +153,41 @@ class _Cpu:
             self._make_recipe_state(_VP(base + i * item), i, seed, sigma, t_min, t_max)
         return cells
 
+    def set_lu_precision(self, mode, sweeps=2):
+        """0: FP64 factors; 1: FP32 factors + FP64 refinement (oracle global)."""
+        self._set_lu_precision(int(mode), int(sweeps))
+        self._reset_mixed_fallbacks()
+
+    def lu32(self, a):
+        a = np.array(a, dtype=np.float32).reshape(15, 15).copy()
+        ip = np.zeros(15, dtype=np.int32)
+        info = self._lu32(_p(a), _p(ip))
+        return a, ip, info
+
+    def lu_solve32(self, lu, ip, x):
+        x = np.array(x, dtype=np.float32).copy()
+        self._lu_solve32(_p(np.ascontiguousarray(lu, dtype=np.float32)),
+                         _p(np.ascontiguousarray(ip, dtype=np.int32)), _p(x))
+        return x
+
+    def solve_mixed(self, jac, fac, y, b, sweeps=2):
+        x = np.array(b, dtype=np.float64).copy()
+        r = self._solve_mixed(_p(np.ascontiguousarray(jac, dtype=np.float64)), float(fac),
+                              _p(np.ascontiguousarray(y, dtype=np.float64)), int(sweeps), _p(x))
+        return x, r
+
+    def mixed_lu(self, jac, fac, y):
+        lu = np.zeros(225, dtype=np.float32)
+        ip = np.zeros(15, dtype=np.int32)
+        info = self._mixed_lu(_p(np.ascontiguousarray(jac, dtype=np.float64)), float(fac),
+                              _p(np.ascontiguousarray(y, dtype=np.float64)), _p(lu), _p(ip))
+        return lu, ip, info
+
+    def mixed_fallbacks(self):
+        out = np.zeros(2, dtype=np.uint64)
+        self._mixed_fallbacks(_p(out[:1]), _p(out[1:]))
+        return int(out[0]), int(out[1])
+
     def run_gridwide(self, cells, perturb=True, max_steps=0):
         done = _I(0)
         r = self._run_gridwide(_p(cells), len(cells), int(perturb), max_steps, ctypes.byref(done))
@@ -185,3 +226,115 @@ class Reference(_Cpu):
         if not os.path.exists(REF_SO):
             return None
         return cls(REF_SO)
+
+
+class OracleAdjoint(OracleWrMath):
+    """The oracle with the tangent-linear and adjoint restatement (adjoint.c),
+    evaluated with the GPU path's libm."""
+
+    def __init__(self):
+        path = os.path.join(REPO, "oracle", "_build", "liboracle_adj.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (make -C oracle)")
+        _Cpu.__init__(self, path)
+        L = self.lib
+        sig = {
+            "or_adj_ntheta": (_I, []),
+            "or_lu_solve_t": (None, [_VP, _VP, _VP]),
+            "or_adj_record": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
+            "or_adj_replay": (_I, [_VP, _VP, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
+            "or_adj_backward": (_I, [_VP, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
+            "or_adj_f": (None, [_VP, _VP, _VP]),
+            "or_adj_fj": (None, [_VP, _VP, _VP, _VP]),
+            "or_adj_f_jvp": (None, [_VP, _VP, _VP, _VP, _VP]),
+            "or_adj_f_vjp": (None, [_VP, _VP, _VP, _VP, _VP]),
+            "or_adj_jac_jvp": (None, [_VP, _VP, _VP]),
+            "or_adj_jac_vjp": (None, [_VP, _VP, _VP]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        self.R = L.or_adj_ntheta()
+
+    def lu_solve_t(self, lu, ip, x):
+        x = np.array(x, dtype=np.float64).copy()
+        self.lib.or_lu_solve_t(_p(np.ascontiguousarray(lu, dtype=np.float64)),
+                               _p(np.ascontiguousarray(ip, dtype=np.int32)), _p(x))
+        return x
+
+    def f(self, y, M):
+        out = np.zeros(15)
+        self.lib.or_adj_f(_p(np.ascontiguousarray(y, dtype=np.float64)),
+                          _p(np.ascontiguousarray(M, dtype=np.float64)), _p(out))
+        return out
+
+    def fj(self, y, M):
+        f, J = np.zeros(15), np.zeros(225)
+        self.lib.or_adj_fj(_p(np.ascontiguousarray(y, dtype=np.float64)),
+                           _p(np.ascontiguousarray(M, dtype=np.float64)), _p(f), _p(J))
+        return f, J
+
+    def f_jvp(self, y, M, dy, dM):
+        out = np.zeros(15)
+        self.lib.or_adj_f_jvp(*[_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (y, M, dy, dM)],
+                              _p(out))
+        return out
+
+    def f_vjp(self, y, M, w):
+        yb, mb = np.zeros(15), np.zeros(self.R)
+        self.lib.or_adj_f_vjp(*[_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (y, M, w)],
+                              _p(yb), _p(mb))
+        return yb, mb
+
+    def jac_jvp(self, y, dy):
+        out = np.zeros(225)
+        self.lib.or_adj_jac_jvp(_p(np.ascontiguousarray(y, dtype=np.float64)),
+                                _p(np.ascontiguousarray(dy, dtype=np.float64)), _p(out))
+        return out
+
+    def jac_vjp(self, y, W):
+        out = np.zeros(15)
+        self.lib.or_adj_jac_vjp(_p(np.ascontiguousarray(y, dtype=np.float64)),
+                                _p(np.ascontiguousarray(W, dtype=np.float64)), _p(out))
+        return out
+
+    def record(self, cells, nsteps, M, hcap=512):
+        """Forward run with multipliers M recording the tape; returns a dict."""
+        n = len(cells)
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        t = {"n": n, "nsteps": nsteps, "hcap": hcap, "M": M,
+             "dts": np.zeros(nsteps), "ckpt": np.zeros((nsteps, 18, n)),
+             "hoff": np.zeros((nsteps + 1, n), dtype=np.int32), "htape": np.zeros((n, hcap))}
+        r = self.lib.or_adj_record(_p(cells), n, nsteps, _p(M), _p(t["dts"]), _p(t["ckpt"]),
+                                   _p(t["hoff"]), _p(t["htape"]), hcap)
+        cells["reserved_"] = 0
+        t["status"] = r
+        return t
+
+    def replay(self, t, s0, M=None, ds0=None, dM=None):
+        """Frozen map from state s0 [18, n]; with tangents when ds0 is given."""
+        n, ns = t["n"], t["nsteps"]
+        M = t["M"] if M is None else np.ascontiguousarray(M, dtype=np.float64)
+        s0 = np.ascontiguousarray(s0, dtype=np.float64)
+        sN = np.zeros((18, n))
+        dsN = np.zeros((18, n)) if ds0 is not None else None
+        if ds0 is not None:
+            ds0 = np.ascontiguousarray(ds0, dtype=np.float64)
+            dM = np.ascontiguousarray(dM if dM is not None else np.zeros(self.R), dtype=np.float64)
+        r = self.lib.or_adj_replay(_p(s0), _p(ds0) if ds0 is not None else None, n, ns,
+                                   _p(t["dts"]), _p(t["hoff"]), _p(t["htape"]), t["hcap"], _p(M),
+                                   _p(dM) if ds0 is not None else None, _p(sN),
+                                   _p(dsN) if ds0 is not None else None)
+        assert r == 1, r
+        return (sN, dsN) if ds0 is not None else sN
+
+    def backward(self, t, gN):
+        n, ns = t["n"], t["nsteps"]
+        g0 = np.zeros((18, n))
+        mb = np.zeros((self.R, n))
+        r = self.lib.or_adj_backward(_p(t["ckpt"]), n, ns, _p(t["dts"]), _p(t["hoff"]),
+                                     _p(t["htape"]), t["hcap"], _p(t["M"]),
+                                     _p(np.ascontiguousarray(gN, dtype=np.float64)), _p(g0), _p(mb))
+        assert r == 1, r
+        return g0, mb

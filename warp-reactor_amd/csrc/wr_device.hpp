@@ -397,10 +397,11 @@ __device__ __forceinline__ int piv_get(uint64_t piv, int k) { return (int)((piv 
 
 // Row exchange k <-> p of a register vector (p >= k, dynamic), with selects;
 // skipped when no lane of the wave exchanges.
-__device__ __forceinline__ void swap_rows(double (&v)[kNeqs], int k, int p) {
+template <typename R>
+__device__ __forceinline__ void swap_rows(R (&v)[kNeqs], int k, int p) {
     if (__ballot(p != k) == 0) return;
-    const double vk = v[k];
-    double vp = vk;
+    const R vk = v[k];
+    R vp = vk;
 #pragma unroll
     for (int i = 0; i < kNeqs; ++i)
         if (i > k) vp = (p == i) ? v[i] : vp;
@@ -421,31 +422,33 @@ __device__ __forceinline__ void swap_rows(double (&v)[kNeqs], int k, int p) {
 // 4 bits per column in *piv.
 constexpr int kCache = 5;
 
-__device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& piv) {
-    const int st = w.stride;
-    const double* __restrict__ J = w.J;
-    double* __restrict__ LU = w.LU;
+// R = double: E itself.  R = float: E rounded to float after the exact
+// power-of-two equilibration E_ij 2^(ks_j - ks_i) (ks: mixed_scales).
+template <typename R>
+__device__ __forceinline__ int lu_left_t(const double* __restrict__ J, R* __restrict__ LU, int st,
+                                         double fac, uint64_t& piv, const int* ks = nullptr) {
     int info = 0;
     uint32_t sing = 0;
     piv = 0;
-    double mc[kCache][kNeqs];
+    R mc[kCache][kNeqs];
 #pragma unroll
     for (int j = 0; j < kNeqs; ++j) {
-        double col[kNeqs];
+        R col[kNeqs];
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) {
             const double v = -WR_AT(J, i * kNeqs + j, st);
-            col[i] = (i == j) ? v + fac : v;
+            const double eij = (i == j) ? v + fac : v;
+            col[i] = ks ? static_cast<R>(ldexp(eij, ks[j] - ks[i])) : static_cast<R>(eij);
         }
 #pragma unroll
         for (int k = 0; k < kNeqs - 1; ++k) {
             if (k < j && !((sing >> k) & 1u)) {
                 swap_rows(col, k, piv_get(piv, k));
-                const double t = col[k];
+                const R t = col[k];
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
                     if (i > k) {
-                        const double m =
+                        const R m =
                             (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
                         col[i] += t * m;
                     }
@@ -454,27 +457,27 @@ __device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& 
         }
         if (j < kNeqs - 1) {
             int p = j;
-            double best = fabs(col[j]);
+            R best = fabs(col[j]);
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i) {
                 if (i <= j) continue;
-                const double v = fabs(col[i]);
+                const R v = fabs(col[i]);
                 if (v > best) {
                     best = v;
                     p = i;
                 }
             }
             piv |= (uint64_t)p << (4 * j);
-            double pv = col[j];
+            R pv = col[j];
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i)
                 if (i > j) pv = (p == i) ? col[i] : pv;
-            if (pv == 0.0) {
+            if (pv == R(0)) {
                 info = j + 1;
                 sing |= 1u << j;
             } else {
                 swap_rows(col, j, p);
-                const double mult = -1.0 / col[j];
+                const R mult = R(-1) / col[j];
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i)
                     if (i > j) col[i] *= mult;
@@ -485,7 +488,7 @@ __device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& 
             }
         } else {
             piv |= (uint64_t)(kNeqs - 1) << (4 * j);
-            if (col[kNeqs - 1] == 0.0) info = kNeqs;
+            if (col[kNeqs - 1] == R(0)) info = kNeqs;
         }
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) WR_AT(LU, i * kNeqs + j, st) = col[i];
@@ -493,14 +496,18 @@ __device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& 
     return info;
 }
 
+__device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& piv) {
+    return lu_left_t<double>(w.J, w.LU, w.stride, fac, piv);
+}
+
 // lu_solve (reproducer.cpp:154-181) from the workspace factors, x in registers.
-__device__ __forceinline__ void lu_solve_ws(const ThreadWs& w, uint64_t piv, double (&x)[kNeqs]) {
-    const int st = w.stride;
-    const double* __restrict__ LU = w.LU;
+template <typename R>
+__device__ __forceinline__ void lu_solve_t(const R* __restrict__ LU, int st, uint64_t piv,
+                                           R (&x)[kNeqs]) {
 #pragma unroll
     for (int k = 0; k < kNeqs - 1; ++k) {
         swap_rows(x, k, piv_get(piv, k));
-        const double t = x[k];
+        const R t = x[k];
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
             if (j > k) x[j] += t * WR_AT(LU, j * kNeqs + k, st);
@@ -509,11 +516,84 @@ __device__ __forceinline__ void lu_solve_ws(const ThreadWs& w, uint64_t piv, dou
     for (int kb = 0; kb < kNeqs; ++kb) {
         const int k = kNeqs - 1 - kb;
         x[k] /= WR_AT(LU, k * kNeqs + k, st);
-        const double t = -x[k];
+        const R t = -x[k];
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
             if (j < k) x[j] += t * WR_AT(LU, j * kNeqs + k, st);
     }
+}
+
+__device__ __forceinline__ void lu_solve_ws(const ThreadWs& w, uint64_t piv, double (&x)[kNeqs]) {
+    lu_solve_t<double>(w.LU, w.stride, piv, x);
+}
+
+// ---- mixed precision: FP32 factors, FP64 residual, iterative refinement ------------
+// The unknowns are equilibrated by powers of two taken from the controller's
+// weights, s_i = 2^ks_i with ks_i = ilogb(atol_i + rtol_i |y_i|) (exact scaling:
+// the species are number densities of 1e-40..1, the energy ~1e10): the FP32
+// factors are those of D^-1 E D, D = diag(s), each element built in FP64
+// exactly as the FP64 path builds E, scaled, then rounded.  A stage solve
+// E x = b takes x = D E32^-1 D^-1 b and then `sweeps` refinement sweeps
+// x += D E32^-1 D^-1 (b - E x), the residual in FP64 from J:
+// r_i = b_i - sum_j E_ij x_j, j ascending.  The solve has converged when the
+// last correction is at most 1e-6 s-weights (atol_i + rtol_i |y_i|) in every
+// component; a non-finite correction never converges.
+constexpr double kRefineTol = 1.0e-6;
+
+__device__ __forceinline__ void mixed_scales(const ThreadWs& w, int (&ks)[kNeqs]) {
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i)
+        ks[i] = ilogb(atol_of(i) + rtol_of(i) * fabs(WR_AT(w.vec, kVecY + i, w.stride)));
+}
+
+__device__ __forceinline__ void residual_e(const double* __restrict__ J, int st, double fac,
+                                           const double (&b)[kNeqs], const double (&x)[kNeqs],
+                                           double (&r)[kNeqs]) {
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j) {
+            const double v = -WR_AT(J, i * kNeqs + j, st);
+            const double eij = (i == j) ? v + fac : v;
+            acc += eij * x[j];
+        }
+        r[i] = b[i] - acc;
+    }
+}
+
+// x (in: b, out: the refined solution); returns whether the refinement converged.
+__device__ __forceinline__ bool solve_mixed(const ThreadWs& w, const float* __restrict__ LU32,
+                                            uint64_t piv32, const int (&ks)[kNeqs], double fac,
+                                            int sweeps, double (&x)[kNeqs]) {
+    const int st = w.stride;
+    double b[kNeqs];
+    float xf[kNeqs];
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) {
+        b[i] = x[i];
+        xf[i] = static_cast<float>(ldexp(x[i], -ks[i]));
+    }
+    lu_solve_t<float>(LU32, st, piv32, xf);
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) x[i] = ldexp(static_cast<double>(xf[i]), ks[i]);
+    bool conv = true;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        double r[kNeqs];
+        residual_e(w.J, st, fac, b, x, r);
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) xf[i] = static_cast<float>(ldexp(r[i], -ks[i]));
+        lu_solve_t<float>(LU32, st, piv32, xf);
+        conv = true;
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) {
+            const double d = ldexp(static_cast<double>(xf[i]), ks[i]);
+            x[i] += d;
+            const double sk = atol_of(i) + rtol_of(i) * fabs(WR_AT(w.vec, kVecY + i, st));
+            conv = conv && (fabs(d) <= kRefineTol * sk);
+        }
+    }
+    return conv;
 }
 
 }  // namespace wr

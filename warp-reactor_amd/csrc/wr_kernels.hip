@@ -75,6 +75,9 @@ struct Ctl {
     int queue;          // next queue position to hand out in the queue-fed advance
     int queue_solo;     // next of the positions reserved for the solo wave
     int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
+    int lu_sweeps;        // refinement sweeps per stage solve (FP32-factorisation form)
+    unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
+    unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
     int bin_count[16];  // cells per attempt-count bin, for the queue order
     int bin_used[16];   // queue positions already handed to blocks, per bin
 };
@@ -416,6 +419,11 @@ constexpr int kPrivLen = 2 * kNeqs * kNeqs + kVecLen + kStageSlots;
 // three other waves while the bulk of the cells is being advanced.
 constexpr int kSoloLanes = 64;
 
+// kMixed: the FP32-factorisation form (E = fac I - J factorised in FP32, FP64
+// residual and refinement sweeps per stage solve, FP64 factors when the FP32
+// ones are singular or a refinement does not converge); the RHS, energy
+// equation, error norm and controller stay FP64.
+template <bool kMixed>
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero) {
     using C = Ros2sCoef;
     if (ctl->done) return;
@@ -423,6 +431,9 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     if (solo && threadIdx.x >= kSoloLanes) return;
     double priv[kPrivLen];
     double* const p = priv + zero;
+    float priv32[kMixed ? kNeqs * kNeqs : 1];
+    float* const lu32 = priv32 + zero;
+    const int sweeps = ctl->lu_sweeps;
     ThreadWs w;
     w.J = p;
     w.LU = p + kNeqs * kNeqs;
@@ -439,7 +450,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     int nsing = 0, n_step = 0, n_accept = 0;
     bool reject = false, last = false, top = true;
     Counters cnt{0, 0, 0, 0, 0, 0, 0};
-    uint64_t piv = 0;
+    uint64_t piv = 0, piv32 = 0;
     for (;;) {
         if (c < 0) {
             // ---- take the next cell: advance_collapse_gridwide_kernel's preamble
@@ -527,7 +538,37 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
         if (result == 0) {
             // retry loop body (7260-7345): factorize, three stages, controller
             const double fac = 1.0 / (h * C::gamma);
-            if (lu_left(w, fac, piv) != 0) {
+            bool use64 = !kMixed;
+            int info;
+            int ks[kNeqs];
+            if (kMixed) {
+                mixed_scales(w, ks);
+                info = lu_left_t<float>(w.J, lu32, st, fac, piv32, ks);
+                if (info != 0) {  // singular in FP32: FP64 factors for this attempt
+                    atomicAdd(&ctl->mixed_lu_fallbacks, 1ull);
+                    info = lu_left(w, fac, piv);
+                    use64 = true;
+                }
+            } else {
+                info = lu_left(w, fac, piv);
+            }
+            // stage solve: FP64 factors, or FP32 factors with refinement and the
+            // FP64 factors on first non-convergence
+            auto solve = [&](double (&v)[kNeqs]) {
+                if (!use64) {
+                    double b[kNeqs];
+#pragma unroll
+                    for (int i = 0; i < kNeqs; ++i) b[i] = v[i];
+                    if (solve_mixed(w, lu32, piv32, ks, fac, sweeps, v)) return;
+                    atomicAdd(&ctl->mixed_refine_fallbacks, 1ull);
+                    if (lu_left(w, fac, piv) != 0) return;  // keep the refined FP32 result
+                    use64 = true;
+#pragma unroll
+                    for (int i = 0; i < kNeqs; ++i) v[i] = b[i];
+                }
+                lu_solve_ws(w, piv, v);
+            };
+            if (info != 0) {
                 nsing += 1;
                 if (nsing >= 5) {
                     result = kLuError;
@@ -541,7 +582,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                 double v[kNeqs];
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) v[i] = WR_AT(w.vec, kVecF0 + i, st);
-                lu_solve_ws(w, piv, v);  // ak1
+                solve(v);  // ak1
                 cnt.n_solve += 1;
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
@@ -553,7 +594,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i)
                     v[i] = WR_AT(w.vec, kVecAk2 + i, st) + WR_AT(w.vec, kVecDy + i, st);
-                lu_solve_ws(w, piv, v);  // ak2
+                solve(v);  // ak2
                 cnt.n_solve += 1;
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
@@ -567,7 +608,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i)
                     v[i] = WR_AT(w.vec, kVecAk3 + i, st) + WR_AT(w.vec, kVecDy + i, st);
-                lu_solve_ws(w, piv, v);  // ak3
+                solve(v);  // ak3
                 cnt.n_solve += 1;
                 double err = 0.0;
 #pragma unroll
@@ -729,6 +770,43 @@ __global__ void k_eval_solve(const double* __restrict__ lu, const int* __restric
     for (int k = 0; k < kNeqs; ++k) x[i * kNeqs + k] = xx[k];
 }
 
+// One stage solve of the FP32-factorisation form per thread (the queue-fed
+// kernel's functions): E = fac I - J equilibrated and factorised in FP32, then
+// x solved with `sweeps` refinement sweeps.  conv: -1 FP32 factors singular,
+// else whether the refinement converged; lu_out: the FP32 factors.
+__global__ void k_eval_solve_mixed(const double* __restrict__ jac, const double* __restrict__ fac,
+                                   const double* __restrict__ y, int n, int sweeps, int zero,
+                                   double* __restrict__ x, int* __restrict__ conv,
+                                   float* __restrict__ lu_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double priv[kNeqs * kNeqs + kVecLen];
+    double* const p = priv + zero;
+    float priv32[kNeqs * kNeqs];
+    float* const lu32 = priv32 + zero;
+    ThreadWs w;
+    w.J = p;
+    w.LU = nullptr;
+    w.vec = p + kNeqs * kNeqs;
+    w.stage = nullptr;
+    w.stride = 1;
+    for (int k = 0; k < kNeqs * kNeqs; ++k) w.J[k] = jac[static_cast<size_t>(i) * kNeqs * kNeqs + k];
+    for (int k = 0; k < kNeqs; ++k) w.vec[kVecY + k] = y[i * kNeqs + k];
+    int ks[kNeqs];
+    mixed_scales(w, ks);
+    uint64_t piv32 = 0;
+    const int info = lu_left_t<float>(w.J, lu32, 1, fac[i], piv32, ks);
+    for (int k = 0; k < kNeqs * kNeqs; ++k) lu_out[static_cast<size_t>(i) * kNeqs * kNeqs + k] = lu32[k];
+    double v[kNeqs];
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) v[k] = x[i * kNeqs + k];
+    int c = -1;
+    if (info == 0) c = solve_mixed(w, lu32, piv32, ks, fac[i], sweeps, v) ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < kNeqs; ++k) x[i * kNeqs + k] = v[k];
+    conv[i] = c;
+}
+
 // Device libm on a batch: 0 exp, 1 log, 2 cbrt, 3 sqrt as the network uses
 // them (wr_math.h / IEEE sqrt), 4..6 the ROCm device library's exp, log, cbrt.
 __global__ void k_eval_math(int op, const double* __restrict__ x, int n, double* __restrict__ out) {
@@ -827,6 +905,9 @@ struct wr_context {
     void* ws_mem = nullptr;
     // advance kernel: 1 (default) queue-fed persistent kernel, 0 one thread per cell
     int advance_kernel = 1;
+    // factorisation of the queue-fed kernel: 0 FP64, 1 FP32 + FP64 refinement
+    int lu_mode = 0;
+    int lu_sweeps = 2;
     int integ_max_steps = kMaxSteps;
     int q_blocks = 0;
     // HIP-event timing of the advance launches of wr_run (when enabled)
@@ -979,6 +1060,7 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.failed_step = -1;
     h.limiting_cell = -1;
     h.integ_max_steps = ctx->integ_max_steps;
+    h.lu_sweeps = ctx->lu_sweeps;
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
            "hipMemcpy(control to device)");
@@ -1014,7 +1096,8 @@ static int queue_grid(wr_context* ctx) {
     int cus = 0, per_cu = 0;
     WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
            "hipDeviceGetAttribute(CU count)");
-    WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_advance_q, kQBlock, 0),
+    WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+               &per_cu, ctx->lu_mode ? k_advance_q<true> : k_advance_q<false>, kQBlock, 0),
            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (per_cu < 1) per_cu = 1;
     int blocks = cus * per_cu;
@@ -1025,6 +1108,9 @@ static int queue_grid(wr_context* ctx) {
 }
 
 static int launch_advance(wr_context* ctx) {
+    if (ctx->lu_mode != 0 && ctx->advance_kernel != 1)
+        return fail(WR_ERR_ARGUMENT,
+                    "FP32 factorisation (wr_set_lu_precision) needs the queue-fed advance kernel");
     if (ctx->advance_kernel == 1) {
         const int st = queue_grid(ctx);
         if (st != WR_SUCCESS) return st;
@@ -1033,8 +1119,12 @@ static int launch_advance(wr_context* ctx) {
                            ctx->ctl);
         hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
                            ctx->ctl);
-        hipLaunchKernelGGL(k_advance_q, dim3(ctx->q_blocks), dim3(kQBlock), 0, ctx->stream,
-                           ctx->cells, ctx->ctl, 0);
+        if (ctx->lu_mode)
+            hipLaunchKernelGGL(k_advance_q<true>, dim3(ctx->q_blocks), dim3(kQBlock), 0,
+                               ctx->stream, ctx->cells, ctx->ctl, 0);
+        else
+            hipLaunchKernelGGL(k_advance_q<false>, dim3(ctx->q_blocks), dim3(kQBlock), 0,
+                               ctx->stream, ctx->cells, ctx->ctl, 0);
     } else {
         hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
                            ctx->stream, ctx->cells, ctx->ws, ctx->ctl);
@@ -1082,7 +1172,8 @@ static int enqueue_steps(wr_context* ctx, int todo, bool timed) {
 // except the cell count and the advance kernel, so the graph is captured
 // again only when those change.  The graph holds kernel nodes only.
 static int chunk_graph(wr_context* ctx) {
-    if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == ctx->advance_kernel)
+    const int kernel_key = ctx->advance_kernel + 2 * ctx->lu_mode;
+    if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == kernel_key)
         return WR_SUCCESS;
     if (ctx->chunk_exec) {
         WR_HIP(hipGraphExecDestroy(ctx->chunk_exec), "hipGraphExecDestroy");
@@ -1104,7 +1195,7 @@ static int chunk_graph(wr_context* ctx) {
     (void)hipGraphDestroy(g);
     WR_HIP(ie, "hipGraphInstantiate");
     ctx->graph_n = ctx->n;
-    ctx->graph_kernel = ctx->advance_kernel;
+    ctx->graph_kernel = kernel_key;
     return WR_SUCCESS;
 }
 
@@ -1175,6 +1266,25 @@ int wr_set_advance_kernel(wr_context* ctx, int kernel) {
     if (!ctx || kernel < 0 || kernel > 1)
         return fail(WR_ERR_ARGUMENT, "wr_set_advance_kernel: kernel must be 0 or 1");
     ctx->advance_kernel = kernel;
+    return WR_SUCCESS;
+}
+
+int wr_set_lu_precision(wr_context* ctx, int mode, int sweeps) {
+    if (!ctx || mode < 0 || mode > 1 || (mode == 1 && (sweeps < 1 || sweeps > 4)))
+        return fail(WR_ERR_ARGUMENT,
+                    "wr_set_lu_precision: mode must be 0 or 1, sweeps 1..4 for mode 1");
+    if (mode != ctx->lu_mode) ctx->q_blocks = 0;  // occupancy of the other kernel
+    ctx->lu_mode = mode;
+    if (mode == 1) ctx->lu_sweeps = sweeps;
+    return WR_SUCCESS;
+}
+
+int wr_mixed_fallbacks(const wr_context* ctx, unsigned long long* lu_fallbacks,
+                       unsigned long long* refine_fallbacks) {
+    if (!ctx || !lu_fallbacks || !refine_fallbacks)
+        return fail(WR_ERR_ARGUMENT, "wr_mixed_fallbacks: bad argument");
+    *lu_fallbacks = ctx->ctl_host->mixed_lu_fallbacks;
+    *refine_fallbacks = ctx->ctl_host->mixed_refine_fallbacks;
     return WR_SUCCESS;
 }
 
@@ -1411,6 +1521,41 @@ int wr_eval_lu_solve(wr_context* ctx, const double* lu, const int32_t* ipvt, int
     WR_HIP(hipGetLastError(), "k_eval_solve launch");
     WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     WR_HIP(hipMemcpy(x, dx.p, bx, hipMemcpyDeviceToHost), "hipMemcpy");
+    return WR_SUCCESS;
+}
+
+int wr_eval_solve_mixed(wr_context* ctx, const double* jac, const double* fac, const double* y,
+                        int n, int sweeps, double* x, int32_t* conv, float* lu32) {
+    int st = eval_common(ctx, n);
+    if (st != WR_SUCCESS || n == 0) return st;
+    if (!jac || !fac || !y || !x || !conv || !lu32 || sweeps < 0 || sweeps > 4)
+        return fail(WR_ERR_ARGUMENT, "wr_eval_solve_mixed: bad argument");
+    DevBuf dj, df, dy, dx, dc, dl;
+    const size_t bj = static_cast<size_t>(n) * kNeqs * kNeqs * sizeof(double);
+    const size_t bv = static_cast<size_t>(n) * kNeqs * sizeof(double);
+    WR_HIP(hipMalloc(&dj.p, bj), "hipMalloc");
+    WR_HIP(hipMalloc(&df.p, static_cast<size_t>(n) * sizeof(double)), "hipMalloc");
+    WR_HIP(hipMalloc(&dy.p, bv), "hipMalloc");
+    WR_HIP(hipMalloc(&dx.p, bv), "hipMalloc");
+    WR_HIP(hipMalloc(&dc.p, static_cast<size_t>(n) * sizeof(int)), "hipMalloc");
+    WR_HIP(hipMalloc(&dl.p, static_cast<size_t>(n) * kNeqs * kNeqs * sizeof(float)), "hipMalloc");
+    WR_HIP(hipMemcpy(dj.p, jac, bj, hipMemcpyHostToDevice), "hipMemcpy");
+    WR_HIP(hipMemcpy(df.p, fac, static_cast<size_t>(n) * sizeof(double), hipMemcpyHostToDevice),
+           "hipMemcpy");
+    WR_HIP(hipMemcpy(dy.p, y, bv, hipMemcpyHostToDevice), "hipMemcpy");
+    WR_HIP(hipMemcpy(dx.p, x, bv, hipMemcpyHostToDevice), "hipMemcpy");
+    hipLaunchKernelGGL(k_eval_solve_mixed, dim3(blocks_for(n, 64)), dim3(64), 0, ctx->stream,
+                       static_cast<double*>(dj.p), static_cast<double*>(df.p),
+                       static_cast<double*>(dy.p), n, sweeps, 0, static_cast<double*>(dx.p),
+                       static_cast<int*>(dc.p), static_cast<float*>(dl.p));
+    WR_HIP(hipGetLastError(), "k_eval_solve_mixed launch");
+    WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    WR_HIP(hipMemcpy(x, dx.p, bv, hipMemcpyDeviceToHost), "hipMemcpy");
+    WR_HIP(hipMemcpy(conv, dc.p, static_cast<size_t>(n) * sizeof(int), hipMemcpyDeviceToHost),
+           "hipMemcpy");
+    WR_HIP(hipMemcpy(lu32, dl.p, static_cast<size_t>(n) * kNeqs * kNeqs * sizeof(float),
+                     hipMemcpyDeviceToHost),
+           "hipMemcpy");
     return WR_SUCCESS;
 }
 

@@ -37,7 +37,7 @@ NUM_SPEC = 14
 NEQS = 15
 
 # Operation codes.  "f" ops produce a double, "b" ops a boolean.
-FLOAT_OPS = {"const", "T", "X", "z", "neg", "abs", "exp", "log", "sqrt",
+FLOAT_OPS = {"const", "T", "X", "z", "in", "neg", "abs", "exp", "log", "sqrt",
              "add", "sub", "mul", "div", "sel"}
 BOOL_OPS = {"bconst", "lt", "le", "gt", "ge", "eq", "ne", "and", "or", "not"}
 ARITY = {"neg": 1, "abs": 1, "exp": 1, "log": 1, "sqrt": 1, "not": 1,
@@ -95,7 +95,7 @@ class Dag:
         return n.val if n.op == "bconst" else None
 
     def _fold(self, op, args, val):
-        if op in ("const", "T", "X", "z", "bconst"):
+        if op in ("const", "T", "X", "z", "bconst", "in"):
             return None
         cv = [self.cval(a) for a in args] if op not in ("and", "or", "not", "sel") else None
         if cv is not None and all(c is not None for c in cv):

@@ -30,7 +30,7 @@ import dag as dagmod
 
 
 def _leaf(d, n):
-    return d.nodes[n].op in ("const", "bconst", "T", "X")
+    return d.nodes[n].op in ("const", "bconst", "T", "X", "in")
 
 
 def cluster_roots(d, roots):
@@ -221,7 +221,7 @@ _INLINE_F64 = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
 
 
 def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT", prefetch=0,
-                prefetch_extra=0, const_table=None, sync_every=0):
+                prefetch_extra=0, const_table=None, sync_every=0, params=None):
     """const_table: name of a table the constants are read from through
     CHEM_K(name, i) (indices in order of first use; values in plan.ktab),
     except those the hardware encodes inline.  sync_every: emit CHEM_SYNC()
@@ -236,7 +236,9 @@ def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT", prefet
         plan.actions = hoist_loads(plan, prefetch, prefetch_extra)
     cur = {}       # node -> current C name
     gen = collections.Counter()
-    lines = [f"CHEM_FN void {fname}(double T, const double* X{sig_extra})", "{"]
+    if params is None:
+        params = f"double T, const double* X{sig_extra}"
+    lines = [f"CHEM_FN void {fname}({params})", "{"]
     kidx = {}
     plan.ktab = []
 
@@ -259,6 +261,9 @@ def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT", prefet
             return "T"
         if n.op == "X":
             return f"X[{n.val}]"
+        if n.op == "in":
+            name, idx = n.val.split(":")
+            return f"{name}[{idx}]"
         return cur[nid]
 
     def expr(nid):

@@ -82,6 +82,9 @@ _SIGS = {
     "wr_set_timing": (_I, [_VP, _I]),
     "wr_set_integrator_max_steps": (_I, [_VP, _I]),
     "wr_set_graph": (_I, [_VP, _I]),
+    "wr_set_lu_precision": (_I, [_VP, _I, _I]),
+    "wr_mixed_fallbacks": (_I, [_VP, ctypes.POINTER(ctypes.c_ulonglong),
+                                ctypes.POINTER(ctypes.c_ulonglong)]),
     "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_eval_rhs": (_I, [_VP, _VP, _I, _VP]),
     "wr_eval_math": (_I, [_VP, _I, _VP, _I, _VP]),
@@ -89,6 +92,7 @@ _SIGS = {
     "wr_eval_lu": (_I, [_VP, _VP, _I, _VP, _VP]),
     "wr_eval_lu_solve": (_I, [_VP, _VP, _VP, _I, _VP]),
     "wr_eval_integrate": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+    "wr_eval_solve_mixed": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP]),
     "wr_make_collapse_states": (_I, [_VP, _I]),
     "wr_make_recipe_states": (_I, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
     "wr_pack_final_states": (_I, [_VP, _I, _I, _VP]),
@@ -252,6 +256,21 @@ class Context:
         """Enqueue outer steps as captured HIP graphs (default) or launches."""
         _check(self._lib.wr_set_graph(self._h, int(bool(enable))), "wr_set_graph")
 
+    def set_lu_precision(self, mode, sweeps=2):
+        """Factorisation in the queue-fed kernel: 0 FP64 (default); 1 FP32
+        factors with FP64 residuals and `sweeps` refinement sweeps per stage
+        solve (FP64 fallback when singular or not converged)."""
+        _check(self._lib.wr_set_lu_precision(self._h, int(mode), int(sweeps)),
+               "wr_set_lu_precision")
+
+    def mixed_fallbacks(self):
+        """(attempts with singular FP32 factors, stage solves whose refinement
+        did not converge) of the last run."""
+        a, b = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        _check(self._lib.wr_mixed_fallbacks(self._h, ctypes.byref(a), ctypes.byref(b)),
+               "wr_mixed_fallbacks")
+        return a.value, b.value
+
     def set_timing(self, enable=True):
         _check(self._lib.wr_set_timing(self._h, int(bool(enable))), "wr_set_timing")
 
@@ -314,6 +333,20 @@ class Context:
         _check(self._lib.wr_eval_lu_solve(self._h, _ptr(lu), _ptr(ipvt), len(x), _ptr(x)),
                "wr_eval_lu_solve")
         return x
+
+    def eval_solve_mixed(self, jac, fac, y, b, sweeps=2):
+        """FP32-factorisation stage solves; returns (x, conv, lu32)."""
+        jac = np.ascontiguousarray(jac, dtype=np.float64).reshape(-1, 225)
+        n = len(jac)
+        fac = np.ascontiguousarray(np.broadcast_to(fac, (n,)), dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(n, 15)
+        x = np.array(b, dtype=np.float64).reshape(n, 15).copy()
+        conv = np.zeros(n, dtype=np.int32)
+        lu = np.zeros((n, 225), dtype=np.float32)
+        _check(self._lib.wr_eval_solve_mixed(self._h, _ptr(jac), _ptr(fac), _ptr(y), n,
+                                             int(sweeps), _ptr(x), _ptr(conv), _ptr(lu)),
+               "wr_eval_solve_mixed")
+        return x, conv, lu
 
     def eval_integrate(self, y, tout, dt0, max_steps=0):
         y = np.array(y, dtype=np.float64).reshape(-1, NEQS).copy()
