@@ -122,6 +122,10 @@ def main():
                          "benchmark recipe (log-normal density, fixed / log-uniform T)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the thread-per-cell comparison run")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the FP32-factorisation run")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint measurement (32^3, 20 outer steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -234,9 +238,102 @@ def main():
                               "tuned_final_states_bit_identical": same,
                               "tuned_max_species_rel_err": float(rel.max()) if rel.size else 0.0}
         print(f"# plain run: {el0:.3f} s, bit-identical {same}", file=sys.stderr, flush=True)
+    if not args.no_mixed and args.advance_kernel == 1:
+        line["mixed_fp32_lu"] = mixed_measure(ctx, one_run, cells, value, n, wr)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_measure(wr)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(line))
+
+
+def rel_errors(a, b, floor=1e-20):
+    """max / median of |a - b| / |b| over entries with |b| > floor."""
+    m = np.abs(b) > floor
+    r = np.abs(a - b)[m] / np.abs(b)[m]
+    return (float(r.max()), float(np.median(r))) if r.size else (0.0, 0.0)
+
+
+def mixed_measure(ctx, one_run, fp64_cells, fp64_value, n, wr):
+    """The same run with FP32 factorisation of I - gamma h J (FP64 residual, two
+    refinement sweeps, FP64 fallback): speed and error against the FP64 run."""
+    ctx.set_lu_precision(1, 2)
+    info, el = one_run()
+    fb = ctx.mixed_fallbacks()
+    ctx.set_lu_precision(0)
+    got = np.zeros(n, dtype=wr.CELL_DTYPE)
+    ctx.download(got)
+    out = {"run_seconds": el, "value": info["cell_advances"] / el,
+           "speed_vs_fp64": (info["cell_advances"] / el) / fp64_value,
+           "fallbacks": {"singular_fp32_factors": fb[0], "unconverged_refinements": fb[1]}}
+    per = {}
+    for k, name in enumerate(wr.SPECIES_NAMES):
+        mx, md = rel_errors(got["current"]["xn"][:, k], fp64_cells["current"]["xn"][:, k])
+        per[name] = [mx, md]
+    out["species_rel_err_max_median"] = per
+    out["T_rel_err_max_median"] = list(rel_errors(got["current"]["T"], fp64_cells["current"]["T"]))
+    for f in ("accepted_steps", "rejected_steps"):
+        out[f"delta_{f}"] = int(got["stats"][f].sum()) - int(fp64_cells["stats"][f].sum())
+    print(f"# mixed run: {el:.3f} s, fallbacks {fb}", file=sys.stderr, flush=True)
+    return out
+
+
+def adjoint_measure(wr, grid=32, nsteps=20, seed=2, directions=8, delta=1e-6):
+    """Discrete adjoint on the uniform16 recipe with the given seed: the
+    production forward run, the recording run and the backward pass (HIP-event
+    and wall times), and the gradient of sum(H2 mass fraction + log T) against
+    central differences of the frozen-map replay along seeded directions."""
+    import torch
+    ag = importlib.import_module("warp-reactor_amd.autograd")
+    n = grid ** 3
+    _, sigma, t0, t1 = wr.RECIPES["uniform16"]
+    init = wr.make_recipe_states(n, seed, sigma, t0, t1)
+    ctx = wr.Context(n)
+    ctx.upload(init)
+    ctx.run(perturb=False, max_steps=1)
+    ctx.upload(init)
+    tf = time.perf_counter()
+    info = ctx.run(perturb=False, max_steps=nsteps)
+    fwd = time.perf_counter() - tf
+    R = wr.adjoint_ntheta()
+    ctx.upload(init)
+    tr = time.perf_counter()
+    ctx.adjoint_record(nsteps, np.ones(R), hcap=1024)
+    rec = time.perf_counter() - tr
+    rec_ms = ctx.adjoint_last_ms()
+    final = np.zeros(n, dtype=wr.CELL_DTYPE)
+    ctx.download(final)
+    sN = torch.tensor(ag.state_of(final), dtype=torch.float64, requires_grad=True)
+    loss = ag.h2_log_t_loss(sN)
+    loss.backward()
+    tb = time.perf_counter()
+    g0, mbar = ctx.adjoint_backward(sN.grad.numpy())
+    bwd = time.perf_counter() - tb
+    bwd_ms = ctx.adjoint_last_ms()
+    gth = mbar.sum(axis=1)  # theta = 0: M = 1
+    s0 = ag.state_of(init)
+    rng = np.random.default_rng(seed)
+    errs = []
+    for _ in range(directions):
+        v = rng.standard_normal(s0.shape) * np.abs(s0)
+        v[15] = 0.0
+        dth = rng.standard_normal(R)
+        lp = ag.h2_log_t_loss(torch.from_numpy(ctx.adjoint_replay(s0 + delta * v, np.exp(delta * dth))))
+        lm = ag.h2_log_t_loss(torch.from_numpy(ctx.adjoint_replay(s0 - delta * v, np.exp(-delta * dth))))
+        fd = (lp.item() - lm.item()) / (2 * delta)
+        an = float(np.sum(g0 * v) + gth @ dth)
+        errs.append(abs(fd - an) / abs(an))
+    ctx.close()
+    print(f"# adjoint: forward {fwd:.3f} s, record {rec:.3f} s, backward {bwd:.3f} s, "
+          f"FD max rel err {max(errs):.2e}", file=sys.stderr, flush=True)
+    return {"workload": f"{grid}^3 = {n} cells, uniform16 recipe seed {seed}, {nsteps} outer steps, "
+                        f"loss sum(H2 mass fraction + log T), {R} log-rate multipliers + 18 state "
+                        "fields per cell",
+            "forward_seconds": fwd, "record_seconds": rec, "backward_seconds": bwd,
+            "record_kernel_ms": rec_ms, "backward_kernel_ms": bwd_ms,
+            "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd,
+            "cell_advances": info["cell_advances"],
+            "fd_check": {"directions": directions, "delta": delta, "max_rel_err": max(errs)}}
 
 
 if __name__ == "__main__":

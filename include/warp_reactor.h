@@ -202,6 +202,39 @@ int wr_set_timing(wr_context *ctx, int enable);
  * that integrated cells, and their count. */
 int wr_last_advance_timing(const wr_context *ctx, double *total_ms, int *launches);
 
+/* ---- discrete adjoint (no counterpart in the reference) ----------------------
+ * Sensitivities of the collapse advance with its controller decisions frozen
+ * at a recorded run: dt_grid of every outer step and the step sizes of every
+ * cell's accepted ROS2S attempts (the forward path is reproducer.cpp:7736-7785,
+ * 7205-7347).  Parameters: each cell's state at its first advance and the rate
+ * multipliers M_r = exp(theta_r) of the network's reaction flux terms (the
+ * species-equation terms of the generated RHS; wr_adjoint_ntheta() of them).
+ * States are [18][n] host arrays, field-major: xn[0..13], T, e, rho,
+ * density_driver.  e of the first state does not enter the map (eos_rt
+ * recomputes it from T).  Perturbations are not part of the map. */
+int wr_adjoint_ntheta(void);
+/* Run `nsteps` outer steps of the uploaded cells with multipliers M (no
+ * perturbation, every cell must integrate in every step) and record the tape:
+ * checkpoints of every step and up to `hcap` accepted step sizes per cell.
+ * The cells' final state stays in the context (wr_download_cells).  Returns
+ * WR_SUCCESS, a failure code, WR_ERR_CAPACITY (tape full) or WR_ERR_ARGUMENT
+ * (a cell stopped integrating). */
+int wr_adjoint_record(wr_context *ctx, int nsteps, const double *multipliers, int hcap,
+                      int *completed_global_steps);
+/* The frozen map from s0 (NULL: the recorded first state) with multipliers
+ * (NULL: the recorded ones) to sN. */
+int wr_adjoint_replay(wr_context *ctx, const double *s0, const double *multipliers, double *sN);
+/* gN = dL/d(final state) -> g0 = dL/d(first state), mbar [R][n] = dL/dM_r per
+ * cell, at the recorded state and multipliers. */
+int wr_adjoint_backward(wr_context *ctx, const double *gN, double *g0, double *mbar);
+/* The recorded tape: hoff [nsteps+1][n] (first tape index of each step),
+ * htape [hcap][n], dts [nsteps], ckpt [nsteps][18][n]; any may be NULL. */
+int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double *dts,
+                    double *ckpt);
+/* Device time (HIP events on the context's stream) of the last record,
+ * replay or backward call, ms. */
+int wr_adjoint_last_ms(const wr_context *ctx, double *ms);
+
 /* ---- device building blocks (batched; for verification) --------------------
  * y: n x 15 row-major host arrays; outputs likewise.  Each runs the same
  * device functions the advance kernel uses. */

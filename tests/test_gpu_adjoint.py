@@ -1,0 +1,148 @@
+"""The discrete adjoint on the GPU against the oracle's restatement
+(oracle/adjoint.c), through the C ABI: the recording run, the tape, the
+frozen-map replay and the backward pass, bit for bit.  Workload: the
+uniform16 recipe with seed 2 (the adjoint benchmark's), 20 outer steps."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 20
+
+
+@pytest.fixture(scope="module")
+def adj():
+    import oracle_lib
+    return oracle_lib.OracleAdjoint()
+
+
+def _cells(wr, adj, n):
+    seed, sigma, t0, t1 = wr.RECIPES["uniform16"]
+    return (wr.make_recipe_states(n, 2, sigma, t0, t1),
+            adj.make_recipe_cells(n, wr.CELL_DTYPE, 2, sigma, t0, t1))
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def _cells_equal(a, b):
+    mask = np.ones(216, bool)
+    mask[156:160] = False
+    return np.array_equal(a.view(np.uint8).reshape(len(a), 216)[:, mask],
+                          b.view(np.uint8).reshape(len(b), 216)[:, mask])
+
+
+@pytest.fixture(scope="module")
+def recorded(wr, adj):
+    n = 4 ** 3
+    cells, ocells = _cells(wr, adj, n)
+    plain = cells.copy()
+    rng = np.random.default_rng(21)
+    M = np.exp(rng.normal(0.0, 0.05, adj.R))
+    print(f"adjoint record: {n} cells x {NSTEPS} steps", flush=True)
+    ctx = wr.Context(n)
+    ctx.upload(cells)
+    done = ctx.adjoint_record(NSTEPS, M, hcap=512)
+    assert done == NSTEPS
+    ctx.download(cells)
+    tape = adj.record(ocells, NSTEPS, M)
+    assert tape["status"] == 1
+    yield ctx, cells, ocells, plain, tape, M
+    ctx.close()
+
+
+def test_record_run_and_tape_match_oracle(recorded):
+    ctx, cells, ocells, _, tape, _ = recorded
+    assert _cells_equal(cells, ocells)
+    t = ctx.adjoint_tape()
+    assert np.array_equal(t["hoff"], tape["hoff"])
+    assert np.array_equal(_bits(t["dts"]), _bits(tape["dts"]))
+    assert np.array_equal(_bits(t["ckpt"]), _bits(tape["ckpt"]))
+    n = len(cells)
+    for c in range(n):
+        k = tape["hoff"][-1][c]
+        assert np.array_equal(_bits(t["htape"][:k, c]), _bits(tape["htape"][c, :k]))
+
+
+def test_record_with_unit_multipliers_is_the_production_run(wr, adj):
+    n = 3 ** 3
+    cells, _ = _cells(wr, adj, n)
+    plain = cells.copy()
+    with wr.Context(n) as ctx:
+        ctx.upload(cells)
+        ctx.adjoint_record(NSTEPS, np.ones(wr.adjoint_ntheta()))
+        ctx.download(cells)
+        ctx.upload(plain)
+        info = ctx.run(perturb=False, max_steps=NSTEPS)
+        ctx.download(plain)
+    assert info["completed_global_steps"] == NSTEPS
+    assert _cells_equal(cells, plain)
+
+
+def test_replay_matches_oracle(recorded, adj):
+    ctx, cells, _, _, tape, M = recorded
+    s0 = tape["ckpt"][0]
+    sN = ctx.adjoint_replay()
+    fin = np.concatenate([cells["current"]["xn"].T, cells["current"]["T"][None],
+                          cells["current"]["e"][None], cells["current"]["rho"][None],
+                          cells["density_driver"][None]])
+    assert np.array_equal(_bits(sN), _bits(fin))
+    rng = np.random.default_rng(22)
+    s1 = s0 * (1.0 + 1e-3 * rng.standard_normal(s0.shape))
+    M1 = M * np.exp(1e-2 * rng.standard_normal(len(M)))
+    got = ctx.adjoint_replay(s1, M1)
+    want = adj.replay(tape, s1, M=M1)
+    assert np.array_equal(_bits(got), _bits(want))
+
+
+def test_backward_matches_oracle(recorded, adj):
+    ctx, _, _, _, tape, _ = recorded
+    rng = np.random.default_rng(23)
+    gN = rng.standard_normal((18, tape["n"]))
+    g0, mb = ctx.adjoint_backward(gN)
+    og0, omb = adj.backward(tape, gN)
+    assert np.array_equal(_bits(g0), _bits(og0))
+    assert np.array_equal(_bits(mb), _bits(omb))
+
+
+def test_autograd_function_gradients(wr, adj):
+    """torch.autograd through the GPU adjoint: the loss sum(H2 mass fraction +
+    log T) of 4^3 cells over 20 steps; theta and s0 gradients against central
+    differences of the GPU frozen-map replay (delta = 1e-6, 8 seeded directions)
+    and against the oracle's backward pass bit for bit."""
+    import importlib
+    import torch
+    ag = importlib.import_module("warp-reactor_amd.autograd")
+    n = 4 ** 3
+    cells, _ = _cells(wr, adj, n)
+    s0 = torch.tensor(ag.state_of(cells), dtype=torch.float64, requires_grad=True)
+    theta = torch.zeros(wr.adjoint_ntheta(), dtype=torch.float64, requires_grad=True)
+    with wr.Context(n) as ctx:
+        sN = ag.collapse(theta, s0, ctx, nsteps=NSTEPS)
+        loss = ag.h2_log_t_loss(sN)
+        loss.backward()
+        gth, gs0 = theta.grad.numpy(), s0.grad.numpy()
+        assert np.all(gs0[15] == 0.0)
+        rng = np.random.default_rng(24)
+        base = s0.detach().numpy()
+        delta = 1e-6
+        for k in range(8):
+            v = rng.standard_normal(base.shape) * np.abs(base)
+            v[15] = 0.0
+            dth = rng.standard_normal(len(gth))
+            lp = ag.h2_log_t_loss(torch.from_numpy(
+                ctx.adjoint_replay(base + delta * v, np.exp(delta * dth)))).item()
+            lm = ag.h2_log_t_loss(torch.from_numpy(
+                ctx.adjoint_replay(base - delta * v, np.exp(-delta * dth)))).item()
+            fd = (lp - lm) / (2 * delta)
+            an = float(np.sum(gs0 * v) + gth @ dth)
+            assert abs(fd - an) <= 1e-5 * abs(an), (k, fd, an)
+    # the same gradients from the oracle's record and backward
+    ocells = adj.make_recipe_cells(n, wr.CELL_DTYPE, 2, *wr.RECIPES["uniform16"][1:])
+    tape = adj.record(ocells, NSTEPS, np.ones(adj.R))
+    sNo = torch.tensor(ag.state_of(ocells), dtype=torch.float64, requires_grad=True)
+    ag.h2_log_t_loss(sNo).backward()
+    og0, omb = adj.backward(tape, sNo.grad.numpy())
+    assert np.array_equal(_bits(og0), _bits(gs0))
+    assert np.array_equal(_bits(omb.sum(axis=1)), _bits(gth))

@@ -16,6 +16,7 @@ from .runtime import (  # noqa: F401
     SPECIES_NAMES,
     Context,
     WarpReactorError,
+    adjoint_ntheta,
     compare_final_states,
     library,
     library_path,

@@ -1,0 +1,85 @@
+"""torch.autograd binding of the discrete adjoint of the collapse advance.
+
+    theta : (R,) log-rate multipliers of the network's reaction flux terms
+            (R = adjoint_ntheta(); M_r = exp(theta_r) scales flux term r in
+            the species equations of the RHS)
+    s0    : (18, n) state of each cell at its first advance, field-major:
+            xn[0..13] (number densities), T, e, rho, density_driver
+    ->  sN: (18, n) state after `nsteps` outer steps (no perturbations)
+
+The forward pass is a recording run on the GPU (the reference integrator on
+f_theta, wr_adjoint_record): it checkpoints every outer step and stores the
+accepted step sizes.  The backward pass (wr_adjoint_backward) differentiates
+that run with its controller decisions frozen -- dt_grid of each step and the
+accepted step sizes -- recomputing each step from its checkpoint and the
+substeps inside it.  e of s0 does not enter the map (the advance recomputes it
+from T), so its gradient is zero.  PyTorch is only the binding: the compute
+runs in the HIP library; tensors cross the boundary as float64 host arrays.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .runtime import CELL_DTYPE, adjoint_ntheta
+
+FIELDS = 18
+
+
+def cells_from_state(s0):
+    """Cells at completed step 0 and time 0 from an (18, n) state array."""
+    s0 = np.asarray(s0, dtype=np.float64)
+    n = s0.shape[1]
+    cells = np.zeros(n, dtype=CELL_DTYPE)
+    cells["current"]["xn"] = s0[:14].T
+    cells["current"]["T"] = s0[14]
+    cells["current"]["e"] = s0[15]
+    cells["current"]["rho"] = s0[16]
+    cells["density_driver"] = s0[17]
+    return cells
+
+
+def state_of(cells):
+    """(18, n) state array of a cell array."""
+    c = cells
+    return np.concatenate([c["current"]["xn"].T, c["current"]["T"][None], c["current"]["e"][None],
+                           c["current"]["rho"][None], c["density_driver"][None]])
+
+
+class CollapseAdjoint(torch.autograd.Function):
+    """sN = collapse(theta, s0) on a wr.Context; gradients by the discrete adjoint."""
+
+    @staticmethod
+    def forward(fctx, theta, s0, context, nsteps, hcap):
+        if theta.shape != (adjoint_ntheta(),):
+            raise ValueError(f"theta must have shape ({adjoint_ntheta()},)")
+        M = torch.exp(theta.detach().to(torch.float64)).cpu().numpy()
+        cells = cells_from_state(s0.detach().to(torch.float64).cpu().numpy())
+        context.upload(cells)
+        context.adjoint_record(nsteps, M, hcap)
+        context.download(cells)
+        fctx.context = context
+        fctx.M = M
+        return torch.from_numpy(state_of(cells)).to(device=s0.device)
+
+    @staticmethod
+    def backward(fctx, gN):
+        g0, mbar = fctx.context.adjoint_backward(gN.detach().to(torch.float64).cpu().numpy())
+        gtheta = mbar.sum(axis=1) * fctx.M  # dM/dtheta = M
+        dev = gN.device
+        return (torch.from_numpy(gtheta).to(device=dev), torch.from_numpy(g0).to(device=dev),
+                None, None, None)
+
+
+def collapse(theta, s0, context, nsteps=20, hcap=512):
+    """Differentiable collapse advance of the cells in s0 (see module doc)."""
+    return CollapseAdjoint.apply(theta, s0, context, nsteps, hcap)
+
+
+H2 = 8  # species index of H2 (reproducer.cpp:228-229)
+H2_MASS = 3.34706503638e-24
+
+
+def h2_log_t_loss(sN):
+    """Sum over cells of the H2 mass fraction m_H2 n_H2 / rho plus log T."""
+    return torch.sum(H2_MASS * sN[H2] / sN[16] + torch.log(sN[14]))

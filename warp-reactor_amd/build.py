@@ -41,10 +41,25 @@ def run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+def run_parallel(cmds):
+    procs = []
+    for cmd in cmds:
+        print("+", " ".join(shlex.quote(c) for c in cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    codes = [p.wait() for p in procs]
+    for cmd, code in zip(cmds, codes):
+        if code != 0:
+            raise subprocess.CalledProcessError(code, cmd)
+
+
 def generate():
     out = os.path.join(CSRC, "_gen", "chem_network.h")
     counts = os.path.join(CSRC, "_gen", "chem_counts.json")
     run([sys.executable, os.path.join(GEN, "emit.py"), out, counts])
+    # derivative code: the device subset (the CPU header is the oracle's)
+    run([sys.executable, os.path.join(GEN, "ad.py"), os.path.join(CSRC, "_gen", "chem_adjoint_cpu.h"),
+         os.path.join(CSRC, "_gen", "chem_adjoint.json"),
+         os.path.join(CSRC, "_gen", "chem_adjoint_dev.h")])
     return out
 
 
@@ -55,11 +70,14 @@ def build_library(extra=()):
     os.makedirs(objdir, exist_ok=True)
     io_obj = os.path.join(objdir, "wr_io.o")
     k_obj = os.path.join(objdir, "wr_kernels.o")
+    a_obj = os.path.join(objdir, "wr_adjoint.o")
     run(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unknown-pragmas",
          "-c", "-o", io_obj, os.path.join(CSRC, "wr_io.cpp")])
-    run([HIPCC, *HIP_FLAGS, *extra, "-c", "-o", k_obj, os.path.join(CSRC, "wr_kernels.hip")])
+    # the two HIP translation units compile concurrently
+    run_parallel([[HIPCC, *HIP_FLAGS, *extra, "-c", "-o", k_obj, os.path.join(CSRC, "wr_kernels.hip")],
+                  [HIPCC, *HIP_FLAGS, *extra, "-c", "-o", a_obj, os.path.join(CSRC, "wr_adjoint.hip")]])
     tmp = LIB + ".tmp"
-    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, k_obj, io_obj])
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, k_obj, a_obj, io_obj])
     os.replace(tmp, LIB)
     return LIB
 

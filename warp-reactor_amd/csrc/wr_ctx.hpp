@@ -1,0 +1,155 @@
+// Device state shared by the library's translation units: the SoA cell
+// state, the per-run control block on the device and the host-side context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/warp_reactor.h"
+
+namespace wr {
+
+// SoA cell state on the device.
+struct DevCells {
+    double* xn;       // [14][cap]
+    double* rho;      // [cap]
+    double* T;
+    double* e;
+    double* time;
+    double* driver;
+    int* completed;
+    unsigned long long* stats;  // [7][cap]
+    int* cost;                  // ROS2S attempts of the cell's last advance
+    int* order;                 // queue position -> cell (most attempts first)
+    int n;
+    int cap;
+};
+
+struct DevWs {
+    double* J;   // [225][cap]
+    double* LU;  // [225][cap]
+    int cap;
+};
+
+// Device control block of one run.
+struct Ctl {
+    double grid_time;
+    double dt_grid;
+    double next_grid_time;
+    unsigned long long cell_advances;
+    int failure;
+    int integrated;
+    int completed;
+    int step;
+    int step_limit;  // the run stops once step reaches this
+    int done;
+    int perturb;
+    int stop_reason;
+    int failed_step;
+    int failed_phase;
+    int limiting_cell;
+    int failed_cell;
+    int step0;          // step index at the start of the run
+    int hist_cap;       // capacity of dt_hist
+    double* dt_hist;    // dt_grid of every outer step of the run (device)
+    int queue;          // next queue position to hand out in the queue-fed advance
+    int queue_solo;     // next of the positions reserved for the solo wave
+    int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
+    int lu_sweeps;        // refinement sweeps per stage solve (FP32-factorisation form)
+    unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
+    unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
+    int bin_count[16];  // cells per attempt-count bin, for the queue order
+    int bin_used[16];   // queue positions already handed to blocks, per bin
+};
+
+// First failure wins (reproducer.cpp:7716, 7724, 7782); the winner records its cell.
+__device__ __forceinline__ void set_failure(Ctl* ctl, int code, int cell) {
+    if (atomicCAS(&ctl->failure, 1, code) == 1) ctl->failed_cell = cell;
+}
+
+constexpr int kMaxStepsDefault = 10000000;  // configure_ros2s (reproducer.cpp:7589)
+
+struct AdjointState;  // tapes and workspaces of the adjoint (wr_adjoint.hip)
+
+}  // namespace wr
+
+namespace wr {
+namespace detail {
+
+extern thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+inline bool hip_ok(hipError_t st, const char* action, std::string* msg = nullptr) {
+    if (st == hipSuccess) return true;
+    const std::string m = std::string(action) + " failed: " + hipGetErrorString(st);
+    g_last_error = m;
+    if (msg) *msg = m;
+    return false;
+}
+
+inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace detail
+}  // namespace wr
+
+#define WR_HIP(call, action)                                          \
+    do {                                                              \
+        if (!::wr::detail::hip_ok((call), (action))) return WR_ERR_HIP; \
+    } while (0)
+
+struct wr_context {
+    int device = 0;
+    int cap = 0;
+    int n = 0;
+    hipStream_t stream = nullptr;
+    wr::DevCells cells{};
+    wr::DevWs ws{};
+    wr::Ctl* ctl = nullptr;         // device
+    wr::Ctl* ctl_host = nullptr;    // pinned mirror
+    static constexpr int kHistCap = 4096;
+    double* dt_hist = nullptr;  // dt_grid per outer step of the last run
+    int hist_len = 0;
+    double* part_dt = nullptr;  // per-prepare-block partials
+    int* part_cell = nullptr;
+    wr_collapse_state* staging = nullptr;  // device AoS staging buffer
+    void* cells_mem = nullptr;
+    void* ws_mem = nullptr;
+    // advance kernel: 1 (default) queue-fed persistent kernel, 0 one thread per cell
+    int advance_kernel = 1;
+    // factorisation of the queue-fed kernel: 0 FP64, 1 FP32 + FP64 refinement
+    int lu_mode = 0;
+    int lu_sweeps = 2;
+    int integ_max_steps = wr::kMaxStepsDefault;
+    int q_blocks = 0;
+    // HIP-event timing of the advance launches of wr_run (when enabled)
+    static constexpr int kChunk = 8;
+    bool timing = false;
+    // one chunk of outer steps captured as a HIP graph (used when not timing)
+    bool use_graph = true;
+    hipGraphExec_t chunk_exec = nullptr;
+    int graph_n = -1, graph_kernel = -1;  // what the captured launches were built for
+    hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
+    double adv_ms = 0.0;
+    int adv_launches = 0;
+    wr_run_info info{};
+    wr::AdjointState* adj = nullptr;  // created on first use by the adjoint entry points
+};
+
+namespace wr {
+namespace detail {
+// Outer-step pieces of wr_kernels.hip used by the adjoint's recording run.
+int ctl_init(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time);
+int ctl_read(wr_context* ctx);
+int ctl_prepare(wr_context* ctx);  // k_prepare + k_plan of one outer step
+int ctl_commit(wr_context* ctx);   // k_commit of one outer step
+void ctl_fill_info(wr_context* ctx);
+void adjoint_destroy(wr_context* ctx);  // wr_adjoint.hip
+}  // namespace detail
+}  // namespace wr
+
+

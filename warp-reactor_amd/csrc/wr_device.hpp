@@ -189,15 +189,30 @@ struct Counters {
     int n_step, n_rhs, n_jac, n_decomp, n_solve, n_accept, n_reject;
 };
 
+// The network an integration evaluates: the generated RHS and Jacobian, and a
+// hook that sees the step size of every accepted attempt.
+struct PlainNet {
+    __device__ __forceinline__ void jac_rhs(const double (&y)[kNeqs], double (&f)[kNeqs],
+                                            double* __restrict__ J, int stride) {
+        wr::jac_rhs(y, f, J, stride);
+    }
+    __device__ __forceinline__ void rhs(const double (&y)[kNeqs], double (&f)[kNeqs]) {
+        wr::rhs(y, f);
+    }
+    __device__ __forceinline__ void accepted(double) {}
+};
+
 // RODAS<PrimordialChem>::integrate from t = 0 to tout (reproducer.cpp:7205-7347).
 // J and LU point at this cell's SoA workspace columns.  The RHS at the start
 // of an attempt, f(y), depends only on y, so it is evaluated together with
 // the Jacobian and reused by every retry of the same step: the reference
 // recomputes it per retry and gets the same bits.
-__device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, double dt0,
-                                               int max_steps, Counters& c,
-                                               double* __restrict__ J, double* __restrict__ LU,
-                                               int stride, double& t_end, double& dt_end) {
+template <class Net>
+__device__ __forceinline__ int ros2s_integrate_net(double (&y)[kNeqs], double tout, double dt0,
+                                                   int max_steps, Counters& c,
+                                                   double* __restrict__ J, double* __restrict__ LU,
+                                                   int stride, double& t_end, double& dt_end,
+                                                   Net& net) {
     using C = Ros2sCoef;
     c = Counters{0, 0, 0, 0, 0, 0, 0};
     if (tout <= 0.0) {
@@ -237,7 +252,7 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
             h = tout - x;
             last = true;
         }
-        jac_rhs(y, f0, J, stride);
+        net.jac_rhs(y, f0, J, stride);
         c.n_jac += 1;
 
         for (;;) {
@@ -275,7 +290,7 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
                 ynew[i] = y[i] + C::a21 * ak1[i];
                 ak2[i] = (C::c21 / h) * ak1[i];
             }
-            rhs(ynew, dy);
+            net.rhs(ynew, dy);
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i) ak2[i] += dy[i];
             lu_solve(LU, stride, ipvt, ak2);
@@ -285,7 +300,7 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
                 ynew[i] = y[i] + C::a31 * ak1[i] + C::a32 * ak2[i];
                 ak3[i] = (C::c31 * ak1[i] + C::c32 * ak2[i]) / h;
             }
-            rhs(ynew, dy);
+            net.rhs(ynew, dy);
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i) ak3[i] += dy[i];
             lu_solve(LU, stride, ipvt, ak3);
@@ -319,6 +334,7 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
                 }
                 hacc = h;
                 erracc = rmax(1.0e-2, err);
+                net.accepted(h);
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) y[i] = ynew[i];
                 x += h;
@@ -334,6 +350,14 @@ __device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, 
             if (n_accept >= 1) c.n_reject += 1;
         }
     }
+}
+
+__device__ __forceinline__ int ros2s_integrate(double (&y)[kNeqs], double tout, double dt0,
+                                               int max_steps, Counters& c,
+                                               double* __restrict__ J, double* __restrict__ LU,
+                                               int stride, double& t_end, double& dt_end) {
+    PlainNet net;
+    return ros2s_integrate_net(y, tout, dt0, max_steps, c, J, LU, stride, t_end, dt_end, net);
 }
 
 // =====================================================================================

@@ -15,6 +15,7 @@
 //              and global step bookkeeping (7943-7958).
 // Every kernel returns immediately once the device control block says the
 // run is done, so the host enqueues several steps ahead and polls one flag.
+#include "wr_ctx.hpp"
 #include "wr_device.hpp"
 
 #include <float.h>
@@ -29,67 +30,10 @@
 
 namespace wr {
 
-// SoA cell state on the device.
-struct DevCells {
-    double* xn;       // [14][cap]
-    double* rho;      // [cap]
-    double* T;
-    double* e;
-    double* time;
-    double* driver;
-    int* completed;
-    unsigned long long* stats;  // [7][cap]
-    int* cost;                  // ROS2S attempts of the cell's last advance
-    int* order;                 // queue position -> cell (most attempts first)
-    int n;
-    int cap;
-};
-
-struct DevWs {
-    double* J;   // [225][cap]
-    double* LU;  // [225][cap]
-    int cap;
-};
-
-// Device control block of one run.
-struct Ctl {
-    double grid_time;
-    double dt_grid;
-    double next_grid_time;
-    unsigned long long cell_advances;
-    int failure;
-    int integrated;
-    int completed;
-    int step;
-    int step_limit;  // the run stops once step reaches this
-    int done;
-    int perturb;
-    int stop_reason;
-    int failed_step;
-    int failed_phase;
-    int limiting_cell;
-    int failed_cell;
-    int step0;          // step index at the start of the run
-    int hist_cap;       // capacity of dt_hist
-    double* dt_hist;    // dt_grid of every outer step of the run (device)
-    int queue;          // next queue position to hand out in the queue-fed advance
-    int queue_solo;     // next of the positions reserved for the solo wave
-    int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
-    int lu_sweeps;        // refinement sweeps per stage solve (FP32-factorisation form)
-    unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
-    unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
-    int bin_count[16];  // cells per attempt-count bin, for the queue order
-    int bin_used[16];   // queue positions already handed to blocks, per bin
-};
-
 constexpr int kPrepBlock = 256;
 constexpr int kAdvBlock = 64;
 constexpr int kPlanBlock = 1024;
 
-// First failure wins (reproducer.cpp:7716, 7724, 7782); the winner records its cell.
-__device__ __forceinline__ void set_failure(Ctl* ctl, int code, int cell) {
-    if (atomicCAS(&ctl->failure, kSuccess, code) == kSuccess) ctl->failed_cell = cell;
-}
 
 // (dt, cell) pair order: smaller dt, then smaller cell.
 __device__ __forceinline__ bool pair_less(double da, int ca, double db, int cb) {
@@ -403,7 +347,6 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(DevCells s, Ctl* 
 // Counters, results and every arithmetic operation are those of the
 // reference; only where values are kept differs.
 constexpr int kQBlock = 256;
-constexpr int kMaxSteps = 10000000;  // configure_ros2s (reproducer.cpp:7589)
 
 // The per-thread workspace is the lane's private memory: the hardware
 // interleaves private (scratch) memory by lane, so element e of all 64 lanes
@@ -860,68 +803,12 @@ __global__ __launch_bounds__(kAdvBlock) void k_eval_integrate(double* __restrict
 // =====================================================================================
 using namespace wr;
 
-namespace {
-
+namespace wr {
+namespace detail {
 thread_local std::string g_last_error;
-
-int fail(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-
-bool hip_ok(hipError_t st, const char* action, std::string* msg = nullptr) {
-    if (st == hipSuccess) return true;
-    const std::string m = std::string(action) + " failed: " + hipGetErrorString(st);
-    g_last_error = m;
-    if (msg) *msg = m;
-    return false;
-}
-
-#define WR_HIP(call, action)                                   \
-    do {                                                       \
-        if (!hip_ok((call), (action))) return WR_ERR_HIP;      \
-    } while (0)
-
-int blocks_for(int n, int b) { return (n + b - 1) / b; }
-
-}  // namespace
-
-struct wr_context {
-    int device = 0;
-    int cap = 0;
-    int n = 0;
-    hipStream_t stream = nullptr;
-    DevCells cells{};
-    DevWs ws{};
-    Ctl* ctl = nullptr;         // device
-    Ctl* ctl_host = nullptr;    // pinned mirror
-    static constexpr int kHistCap = 4096;
-    double* dt_hist = nullptr;  // dt_grid per outer step of the last run
-    int hist_len = 0;
-    double* part_dt = nullptr;  // per-prepare-block partials
-    int* part_cell = nullptr;
-    wr_collapse_state* staging = nullptr;  // device AoS staging buffer
-    void* cells_mem = nullptr;
-    void* ws_mem = nullptr;
-    // advance kernel: 1 (default) queue-fed persistent kernel, 0 one thread per cell
-    int advance_kernel = 1;
-    // factorisation of the queue-fed kernel: 0 FP64, 1 FP32 + FP64 refinement
-    int lu_mode = 0;
-    int lu_sweeps = 2;
-    int integ_max_steps = kMaxSteps;
-    int q_blocks = 0;
-    // HIP-event timing of the advance launches of wr_run (when enabled)
-    static constexpr int kChunk = 8;
-    bool timing = false;
-    // one chunk of outer steps captured as a HIP graph (used when not timing)
-    bool use_graph = true;
-    hipGraphExec_t chunk_exec = nullptr;
-    int graph_n = -1, graph_kernel = -1;  // what the captured launches were built for
-    hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
-    double adv_ms = 0.0;
-    int adv_launches = 0;
-    wr_run_info info{};
-};
+}  // namespace detail
+}  // namespace wr
+using namespace wr::detail;
 
 extern "C" {
 
@@ -946,6 +833,7 @@ void wr_context_destroy(wr_context* ctx) {
         if (ctx->adv_start[k]) (void)hipEventDestroy(ctx->adv_start[k]);
         if (ctx->adv_stop[k]) (void)hipEventDestroy(ctx->adv_stop[k]);
     }
+    wr::detail::adjoint_destroy(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1290,7 +1178,7 @@ int wr_mixed_fallbacks(const wr_context* ctx, unsigned long long* lu_fallbacks,
 
 int wr_set_integrator_max_steps(wr_context* ctx, int max_steps) {
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_set_integrator_max_steps: null context");
-    ctx->integ_max_steps = max_steps > 0 ? max_steps : kMaxSteps;
+    ctx->integ_max_steps = max_steps > 0 ? max_steps : kMaxStepsDefault;
     return WR_SUCCESS;
 }
 
@@ -1595,3 +1483,20 @@ int wr_eval_integrate(wr_context* ctx, double* y, const double* tout, const doub
 }
 
 }  // extern "C"
+
+// ---- outer-step pieces for the adjoint's recording run (wr_adjoint.hip) ----------------
+namespace wr {
+namespace detail {
+int ctl_init(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time) {
+    return ::init_ctl(ctx, perturb, max_steps, completed, grid_time);
+}
+int ctl_read(wr_context* ctx) { return ::read_ctl(ctx); }
+int ctl_prepare(wr_context* ctx) { return ::launch_prepare(ctx); }
+int ctl_commit(wr_context* ctx) {
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
+    WR_HIP(hipGetLastError(), "commit launch");
+    return WR_SUCCESS;
+}
+void ctl_fill_info(wr_context* ctx) { ::fill_info(ctx); }
+}  // namespace detail
+}  // namespace wr

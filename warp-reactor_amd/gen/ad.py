@@ -458,11 +458,11 @@ def emit_staged_fn(d, fname, params, outputs, R, prefetch=500, extra=32):
     for lv, n in outputs:
         outs.setdefault(n, []).append(lv)
     text, plan = stage.emit_staged(d, fname, outs, R, out_macro="AD_OUT", prefetch=prefetch,
-                                   prefetch_extra=extra, params=params)
+                                   prefetch_extra=extra, params=params, in_macro="AD_IN")
     return text, {"slots": plan.nslots, "stores": plan.stores, "loads": plan.loads}
 
 
-def build(out_path, info_path=None):
+def build(out_path, info_path=None, dev_path=None):
     m = Model()
     d, o, R = m.d, m.o, m.R
     dy = [o.inp("dy", i) for i in range(NEQ)]
@@ -531,15 +531,15 @@ def build(out_path, info_path=None):
     info["ops"] = counts
     staged = []
     for name, params, outs, regs in [
-            ("ad_fj_st", f"{P_Y}, const double* M AD_STAGE_PARAMS",
+            ("ad_fj_st", "AD_STAGE_IN(y), AD_STAGE_IN(M) AD_STAGE_PARAMS",
              [(f"F, {i}", m.f[i]) for i in range(NEQ)] +
              [(f"J, {k}", m.J[k]) for k in range(225)], 128),
-            ("ad_f_st", f"{P_Y}, const double* M AD_STAGE_PARAMS",
+            ("ad_f_st", "AD_STAGE_IN(y), AD_STAGE_IN(M) AD_STAGE_PARAMS",
              [(f"F, {i}", m.f[i]) for i in range(NEQ)], 96),
-            ("ad_f_vjp_st", f"{P_Y}, const double* M, const double* w AD_STAGE_PARAMS",
+            ("ad_f_vjp_st", "AD_STAGE_IN(y), AD_STAGE_IN(M), AD_STAGE_IN(w) AD_STAGE_PARAMS",
              [(f"Y, {i}", f_a[i]) for i in range(NEQ)] +
              [(f"MB, {r}", f_a[NEQ + r]) for r in range(R)], 112),
-            ("ad_jac_vjp_st", f"{P_Y}, const double* W AD_STAGE_PARAMS",
+            ("ad_jac_vjp_st", "AD_STAGE_IN(y), AD_STAGE_IN(W) AD_STAGE_PARAMS",
              [(f"Y, {i}", J_a[i]) for i in range(NEQ)], 128)]:
         text, st = emit_staged_fn(d, name, params, outs, regs)
         staged.append(text)
@@ -550,16 +550,25 @@ def build(out_path, info_path=None):
         " * Forward, tangent-linear and adjoint code of one cell's frozen-step collapse",
         " * map: f_theta, J, the advance preamble and the post-step sequence.",
         " * Compile with -ffp-contract=off. The includer defines CHEM_FN, CHEM_EXP,",
-        " * CHEM_LOG; for the staged forms AD_STAGE_PARAMS, STAGE_STORE/LOAD, AD_OUT. */",
+        " * CHEM_LOG; for the staged forms AD_STAGE_IN, AD_IN, AD_STAGE_PARAMS,",
+        " * STAGE_STORE/LOAD and AD_OUT. */",
         f"#define AD_NTHETA {R}",
         f"#define AD_NSLOT {max(nslot, 1)}",
         "",
-    ] + fns + ["", "#ifdef AD_STAGE_PARAMS"] + staged + ["#endif /* AD_STAGE_PARAMS */", ""]
-    os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
-    tmp = out_path + ".tmp"
-    with open(tmp, "w") as f:
-        f.write("\n".join(hdr))
-    os.replace(tmp, out_path)
+    ]
+    # the CPU header: every plain function; the device header: the staged
+    # network pieces and the small plain ones (pre/post and their adjoints)
+    dev_plain = [t for (name, _, _), t in zip(plain, fns)
+                 if name in ("ad_pre", "ad_post", "ad_pre_vjp", "ad_post_vjp")]
+    for path, body in ((out_path, hdr + fns),
+                       (dev_path, hdr + dev_plain + [""] + staged)):
+        if not path:
+            continue
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            f.write("\n".join(body + [""]))
+        os.replace(tmp, path)
     if info_path:
         with open(info_path, "w") as f:
             json.dump(info, f, indent=1)
@@ -567,5 +576,7 @@ def build(out_path, info_path=None):
 
 
 if __name__ == "__main__":
-    inf = build(sys.argv[1], sys.argv[2] if len(sys.argv) > 2 else None)
+    # ad.py CPU_HEADER [INFO_JSON [DEVICE_HEADER]]
+    inf = build(sys.argv[1], sys.argv[2] if len(sys.argv) > 2 else None,
+                sys.argv[3] if len(sys.argv) > 3 else None)
     print("theta terms:", inf["R"], "ops:", inf["ops"], "staged:", inf.get("staged"))

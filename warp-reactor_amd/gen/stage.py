@@ -221,7 +221,7 @@ _INLINE_F64 = {0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0, 4.0, -4.0}
 
 
 def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT", prefetch=0,
-                prefetch_extra=0, const_table=None, sync_every=0, params=None):
+                prefetch_extra=0, const_table=None, sync_every=0, params=None, in_macro=None):
     """const_table: name of a table the constants are read from through
     CHEM_K(name, i) (indices in order of first use; values in plan.ktab),
     except those the hardware encodes inline.  sync_every: emit CHEM_SYNC()
@@ -263,7 +263,7 @@ def emit_staged(d, fname, outputs, R, sig_extra="", out_macro="CHEM_OUT", prefet
             return f"X[{n.val}]"
         if n.op == "in":
             name, idx = n.val.split(":")
-            return f"{name}[{idx}]"
+            return f"{in_macro}({name}, {idx})" if in_macro else f"{name}[{idx}]"
         return cur[nid]
 
     def expr(nid):

@@ -93,6 +93,12 @@ _SIGS = {
     "wr_eval_lu_solve": (_I, [_VP, _VP, _VP, _I, _VP]),
     "wr_eval_integrate": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
     "wr_eval_solve_mixed": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP]),
+    "wr_adjoint_ntheta": (_I, []),
+    "wr_adjoint_record": (_I, [_VP, _I, _VP, _I, ctypes.POINTER(_I)]),
+    "wr_adjoint_replay": (_I, [_VP, _VP, _VP, _VP]),
+    "wr_adjoint_backward": (_I, [_VP, _VP, _VP, _VP]),
+    "wr_adjoint_tape": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "wr_adjoint_last_ms": (_I, [_VP, ctypes.POINTER(_D)]),
     "wr_make_collapse_states": (_I, [_VP, _I]),
     "wr_make_recipe_states": (_I, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
     "wr_pack_final_states": (_I, [_VP, _I, _I, _VP]),
@@ -129,6 +135,11 @@ def _check(code, what):
     if code != SUCCESS:
         msg = library().wr_last_error().decode(errors="replace")
         raise WarpReactorError(code, f"{what}: {msg}")
+
+
+def adjoint_ntheta():
+    """Number of rate multipliers (reaction flux terms) of the adjoint."""
+    return library().wr_adjoint_ntheta()
 
 
 def make_collapse_states(n):
@@ -222,6 +233,7 @@ class Context:
     def upload(self, cells):
         assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
         _check(self._lib.wr_upload_cells(self._h, _ptr(cells), len(cells)), "wr_upload_cells")
+        self._n = len(cells)
 
     def download(self, cells):
         assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
@@ -333,6 +345,53 @@ class Context:
         _check(self._lib.wr_eval_lu_solve(self._h, _ptr(lu), _ptr(ipvt), len(x), _ptr(x)),
                "wr_eval_lu_solve")
         return x
+
+    # ---- discrete adjoint (frozen controller decisions of a recorded run) ----
+    def adjoint_record(self, nsteps, multipliers, hcap=512):
+        """Run nsteps outer steps of the uploaded cells with rate multipliers,
+        recording the tape; returns the completed step count."""
+        M = np.ascontiguousarray(multipliers, dtype=np.float64)
+        done = ctypes.c_int()
+        r = self._lib.wr_adjoint_record(self._h, int(nsteps), _ptr(M), int(hcap), ctypes.byref(done))
+        if r != SUCCESS:
+            _check(r if r <= ERR_HIP else ERR_ARGUMENT, f"wr_adjoint_record (status {r})")
+        self._adj = (int(nsteps), int(hcap))
+        return done.value
+
+    def adjoint_replay(self, s0=None, multipliers=None):
+        n = self._n
+        sN = np.zeros((18, n))
+        a = None if s0 is None else np.ascontiguousarray(s0, dtype=np.float64)
+        m = None if multipliers is None else np.ascontiguousarray(multipliers, dtype=np.float64)
+        r = self._lib.wr_adjoint_replay(self._h, _ptr(a) if a is not None else None,
+                                        _ptr(m) if m is not None else None, _ptr(sN))
+        if r != SUCCESS:
+            _check(r if r <= ERR_HIP else ERR_ARGUMENT, f"wr_adjoint_replay (status {r})")
+        return sN
+
+    def adjoint_backward(self, gN):
+        n = self._n
+        g0 = np.zeros((18, n))
+        mb = np.zeros((adjoint_ntheta(), n))
+        g = np.ascontiguousarray(gN, dtype=np.float64)
+        r = self._lib.wr_adjoint_backward(self._h, _ptr(g), _ptr(g0), _ptr(mb))
+        if r != SUCCESS:
+            _check(r if r <= ERR_HIP else ERR_ARGUMENT, f"wr_adjoint_backward (status {r})")
+        return g0, mb
+
+    def adjoint_tape(self):
+        nsteps, hcap = self._adj
+        n = self._n
+        t = {"hoff": np.zeros((nsteps + 1, n), dtype=np.int32), "htape": np.zeros((hcap, n)),
+             "dts": np.zeros(nsteps), "ckpt": np.zeros((nsteps, 18, n))}
+        _check(self._lib.wr_adjoint_tape(self._h, _ptr(t["hoff"]), _ptr(t["htape"]), _ptr(t["dts"]),
+                                         _ptr(t["ckpt"])), "wr_adjoint_tape")
+        return t
+
+    def adjoint_last_ms(self):
+        ms = ctypes.c_double()
+        _check(self._lib.wr_adjoint_last_ms(self._h, ctypes.byref(ms)), "wr_adjoint_last_ms")
+        return ms.value
 
     def eval_solve_mixed(self, jac, fac, y, b, sweeps=2):
         """FP32-factorisation stage solves; returns (x, conv, lu32)."""
