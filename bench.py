@@ -238,6 +238,7 @@ def main():
                               "tuned_final_states_bit_identical": same,
                               "tuned_max_species_rel_err": float(rel.max()) if rel.size else 0.0}
         print(f"# plain run: {el0:.3f} s, bit-identical {same}", file=sys.stderr, flush=True)
+        ctx.set_advance_kernel(args.advance_kernel)
     if not args.no_mixed and args.advance_kernel == 1:
         line["mixed_fp32_lu"] = mixed_measure(ctx, one_run, cells, value, n, wr)
     if not args.no_adjoint:
@@ -257,6 +258,7 @@ def rel_errors(a, b, floor=1e-20):
 def mixed_measure(ctx, one_run, fp64_cells, fp64_value, n, wr):
     """The same run with FP32 factorisation of I - gamma h J (FP64 residual, two
     refinement sweeps, FP64 fallback): speed and error against the FP64 run."""
+    ctx.set_advance_kernel(1)
     ctx.set_lu_precision(1, 2)
     info, el = one_run()
     fb = ctx.mixed_fallbacks()

@@ -366,6 +366,29 @@ constexpr int kSoloLanes = 64;
 // residual and refinement sweeps per stage solve, FP64 factors when the FP32
 // ones are singular or a refinement does not converge); the RHS, energy
 // equation, error norm and controller stay FP64.
+// Stage solve of the queue-fed kernel: FP64 factors, or (kMixed) FP32 factors
+// with refinement and the FP64 factors from the first non-convergence on.
+template <bool kMixed>
+__device__ __forceinline__ void stage_solve(const ThreadWs& w, const float* lu32, uint64_t piv32,
+                                            const int (&ks)[kNeqs], double fac, int sweeps,
+                                            Ctl* ctl, bool& use64, uint64_t& piv,
+                                            double (&v)[kNeqs]) {
+    if constexpr (kMixed) {
+        if (!use64) {
+            double b[kNeqs];
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) b[i] = v[i];
+            if (solve_mixed(w, lu32, piv32, ks, fac, sweeps, v)) return;
+            atomicAdd(&ctl->mixed_refine_fallbacks, 1ull);
+            if (lu_left(w, fac, piv) != 0) return;  // keep the refined FP32 result
+            use64 = true;
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) v[i] = b[i];
+        }
+    }
+    lu_solve_ws(w, piv, v);
+}
+
 template <bool kMixed>
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero) {
     using C = Ros2sCoef;
@@ -484,7 +507,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
             bool use64 = !kMixed;
             int info;
             int ks[kNeqs];
-            if (kMixed) {
+            if constexpr (kMixed) {
                 mixed_scales(w, ks);
                 info = lu_left_t<float>(w.J, lu32, st, fac, piv32, ks);
                 if (info != 0) {  // singular in FP32: FP64 factors for this attempt
@@ -495,22 +518,6 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
             } else {
                 info = lu_left(w, fac, piv);
             }
-            // stage solve: FP64 factors, or FP32 factors with refinement and the
-            // FP64 factors on first non-convergence
-            auto solve = [&](double (&v)[kNeqs]) {
-                if (!use64) {
-                    double b[kNeqs];
-#pragma unroll
-                    for (int i = 0; i < kNeqs; ++i) b[i] = v[i];
-                    if (solve_mixed(w, lu32, piv32, ks, fac, sweeps, v)) return;
-                    atomicAdd(&ctl->mixed_refine_fallbacks, 1ull);
-                    if (lu_left(w, fac, piv) != 0) return;  // keep the refined FP32 result
-                    use64 = true;
-#pragma unroll
-                    for (int i = 0; i < kNeqs; ++i) v[i] = b[i];
-                }
-                lu_solve_ws(w, piv, v);
-            };
             if (info != 0) {
                 nsing += 1;
                 if (nsing >= 5) {
@@ -525,7 +532,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                 double v[kNeqs];
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) v[i] = WR_AT(w.vec, kVecF0 + i, st);
-                solve(v);  // ak1
+                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak1
                 cnt.n_solve += 1;
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
@@ -537,7 +544,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i)
                     v[i] = WR_AT(w.vec, kVecAk2 + i, st) + WR_AT(w.vec, kVecDy + i, st);
-                solve(v);  // ak2
+                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak2
                 cnt.n_solve += 1;
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i)
                     v[i] = WR_AT(w.vec, kVecAk3 + i, st) + WR_AT(w.vec, kVecDy + i, st);
-                solve(v);  // ak3
+                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak3
                 cnt.n_solve += 1;
                 double err = 0.0;
 #pragma unroll
