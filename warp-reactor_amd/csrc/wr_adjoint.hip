@@ -10,22 +10,23 @@
 // state at the first advance and the rate multipliers M_r = exp(theta_r) of
 // the network's 50 reaction flux terms (the generated f_theta).
 //
-//   k_adj_record   the advance kernel of the recording run: one thread per
-//                  cell, the reference integrator on f_theta, writing the
-//                  cell's state at the start of the advance (checkpoint) and
-//                  the accepted step sizes (tape);
+//   recording      the production queue-fed kernel (wr_queue.hpp) with the
+//                  QThetaNet policy: f_theta, the cell's state at the start of
+//                  each advance (checkpoint) and the accepted step sizes (tape);
 //   k_adj_replay   the frozen map forward from any state / multipliers;
-//   k_adj_backward per cell, steps in reverse: recompute the step from its
-//                  checkpoint storing y at each accepted substep, then reverse
-//                  the post-step sequence, the substeps (recomputing J, LU
-//                  and the stage vectors inside each; transposed LU solves;
-//                  generated adjoints of f_theta and J), and the density
-//                  update.
+//   k_adj_steps    one thread per (cell, step) of a window of steps: recompute
+//                  the step from its checkpoint and store every accepted
+//                  substep's y, LU factors, k1..k3, u2, u3;
+//   k_adj_reverse  one thread per cell, the window's steps in reverse: the
+//                  post-step sequence, the substeps from their stored pieces
+//                  (transposed LU solves, generated adjoints of f_theta and J)
+//                  and the density update.
 // The derivative code is generated from the network DAG (gen/ad.py); staged
 // (register-budgeted) forms write their staging slots and outputs to a
 // per-thread workspace that is structure of arrays over the cells.
 #include "wr_ctx.hpp"
 #include "wr_device.hpp"
+#include "wr_queue.hpp"
 
 #include <string.h>
 
@@ -58,8 +59,8 @@ constexpr int kVF = 0, kVK1 = 15, kVK2 = 30, kVK3 = 45, kVU2 = 60, kVU3 = 75, kV
 constexpr int kWMB = kWV + kVLen;        // accumulated dL/dM of the cell
 constexpr int kWMT = kWMB + kR;          // dL/dM of one f adjoint
 constexpr int kWSTG = kWMT + kR;         // staging slots of the generated code
-constexpr int kWYS = kWSTG + AD_NSLOT;   // y at each accepted substep of one step
-constexpr int kWsFixed = kWYS;
+constexpr int kWsFixed = kWSTG + AD_NSLOT;
+constexpr int kWsSteps = kWSTG + AD_FWD_NSLOT;  // a step-recompute thread's column
 
 struct AdjDev {
     double* ckpt;    // [nsteps][18][cap]
@@ -67,9 +68,9 @@ struct AdjDev {
     double* htape;   // [hcap][cap]: accepted step sizes
     double* dts;     // [nsteps]
     const double* M; // [R]
-    double* ws;      // [kWsFixed + ys_cap * 15][cap]
+    double* ws;      // [kWsFixed][cap]
     int* err;        // first error: -100 a cell did not integrate, -101 tape full
-    int hcap, nsteps, cap, ys_cap;
+    int hcap, nsteps, cap;
 };
 
 __device__ __forceinline__ void adj_error(int* err, int code) { atomicCAS(err, 0, code); }
@@ -84,125 +85,53 @@ __device__ __noinline__ void theta_f(const double* y, const double* M, double* w
                                      double* out) {
     ad_f_st(y, 1, M, 1, w + kWSTG * st, st, out, nullptr);
 }
-__device__ __noinline__ void theta_f_vjp(const double* u, int ust, const double* rb,
-                                         const double* M, double* w, int st) {
-    ad_f_vjp_st(u, ust, M, 1, rb, 1, w + kWSTG * st, st, w + (kWV + kVUB) * st, w + kWMT * st);
-}
-__device__ __noinline__ void theta_jac_vjp(const double* y, double* w, int st) {
-    ad_jac_vjp_st(y, 1, w + kWW * st, st, w + kWSTG * st, st, w + (kWV + kVUB) * st, nullptr);
-}
-
-// ---- the recording run's network: f_theta and J, accepted step sizes to the tape ----
-struct ThetaNet {
+// ---- the recording run in the queue-fed kernel: f_theta, checkpoint, tape ----------------
+struct QThetaNet {
+    static constexpr int kSlots = AD_FWD_NSLOT;
     const double* M;
-    double* w;  // this thread's workspace column
-    int st;     // workspace stride
-    double* htape;
-    int hpos, hcap, cap, c;
-    bool overflow;
-    __device__ __forceinline__ void jac_rhs(const double (&y)[kNeqs], double (&f)[kNeqs],
-                                            double* __restrict__ J, int stride) {
-        theta_fj(y, M, w, st, J);
+    double* ckpt;  // [nsteps][18][cap]
+    int* hoff;     // [nsteps + 1][cap]
+    double* htape; // [hcap][cap]
+    int* err;
+    int hcap, cap, step;
+    int hpos;  // this lane's next tape index
+    __device__ __forceinline__ void jac_rhs(const ThreadWs& w) {
+        double yy[kNeqs];
 #pragma unroll
-        for (int i = 0; i < kNeqs; ++i) f[i] = w[(kWV + kVF + i) * st];
+        for (int i = 0; i < kNeqs; ++i) yy[i] = WR_AT(w.vec, kVecY + i, w.stride);
+        ad_fj_st(yy, 1, M, 1, w.stage, w.stride, w.vec + kVecF0 * w.stride, w.J);
     }
-    __device__ __forceinline__ void rhs(const double (&y)[kNeqs], double (&f)[kNeqs]) {
-        theta_f(y, M, w, st, w + (kWV + kVF) * st);
+    __device__ __forceinline__ void rhs(const ThreadWs& w) {
+        double yy[kNeqs];
 #pragma unroll
-        for (int i = 0; i < kNeqs; ++i) f[i] = w[(kWV + kVF + i) * st];
+        for (int i = 0; i < kNeqs; ++i) yy[i] = WR_AT(w.vec, kVecYnew + i, w.stride);
+        ad_f_st(yy, 1, M, 1, w.stage, w.stride, w.vec + kVecDy * w.stride, nullptr);
     }
-    __device__ __forceinline__ void accepted(double h) {
+    __device__ __forceinline__ void begin(int c, const double (&xn)[kNumSpec], double T, double e,
+                                          double rho, double driver) {
+        double* ck = ckpt + static_cast<size_t>(step) * 18 * cap + c;
+#pragma unroll
+        for (int n = 0; n < kNumSpec; ++n) ck[n * cap] = xn[n];
+        ck[14 * cap] = T;
+        ck[15 * cap] = e;
+        ck[16 * cap] = rho;
+        ck[17 * cap] = driver;
+        hpos = hoff[static_cast<size_t>(step) * cap + c];
+    }
+    __device__ __forceinline__ void skipped(int) { atomicCAS(err, 0, -100); }
+    __device__ __forceinline__ void accepted(int c, double h) {
         if (hpos < hcap)
             htape[static_cast<size_t>(hpos) * cap + c] = h;
         else
-            overflow = true;
+            atomicCAS(err, 0, -101);
         hpos += 1;
     }
+    __device__ __forceinline__ void end(int c) { hoff[static_cast<size_t>(step + 1) * cap + c] = hpos; }
 };
 
-__device__ __noinline__ int integrate_theta(double (&y)[kNeqs], double dt, int max_steps,
-                                            Counters& cnt, ThetaNet& net) {
-    double t_end, dt_end;
-    return ros2s_integrate_net(y, dt, dt, max_steps, cnt, net.w + kWJ * net.st,
-                               net.w + kWLU * net.st, net.st, t_end, dt_end, net);
-}
-
-// advance_collapse_gridwide_kernel (reproducer.cpp:7736-7785) on f_theta, with the
-// checkpoint and the tape.  Every cell must integrate.
-__global__ __launch_bounds__(kAdjBlock) void k_adj_record(DevCells s, Ctl* ctl, AdjDev a, int step) {
-    if (ctl->done) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= s.n || ctl->failure != kSuccess) return;
-    const int completed = ctl->completed;
-    const double dt_grid = ctl->dt_grid;
-    if (threadIdx.x == 0 && blockIdx.x == 0) a.dts[step] = dt_grid;
-    if (s.completed[c] != completed) {
-        adj_error(a.err, -100);
-        return;
-    }
-    double xn[kNumSpec];
-#pragma unroll
-    for (int n = 0; n < kNumSpec; ++n) xn[n] = s.xn[n * s.cap + c];
-    double* ck = a.ckpt + static_cast<size_t>(step) * kNF * a.cap + c;
-#pragma unroll
-    for (int n = 0; n < kNumSpec; ++n) ck[n * a.cap] = xn[n];
-    ck[14 * a.cap] = s.T[c];
-    ck[15 * a.cap] = s.e[c];
-    ck[16 * a.cap] = s.rho[c];
-    ck[17 * a.cap] = s.driver[c];
-    const double local_dt = collapse_timestep(xn);
-    if (local_dt < 10.0) {
-        adj_error(a.err, -100);
-        return;
-    }
-    const double old_density = s.driver[c];
-    const double tff = local_dt / 1.0e-1;
-    const double driver = old_density + dt_grid * (old_density / tff);
-    s.driver[c] = driver;
-    if (!valid_positive(driver) || driver > 2.0e-6) {
-        adj_error(a.err, -100);
-        return;
-    }
-    const double ratio = driver / old_density;
-#pragma unroll
-    for (int n = 0; n < kNumSpec; ++n) {
-        xn[n] *= ratio;
-        s.xn[n * s.cap + c] = xn[n];
-    }
-    const double rho = s.rho[c] * ratio;
-    s.rho[c] = rho;
-    const double e0 = eos_energy(xn, s.T[c]);
-    double y[kNeqs];
-#pragma unroll
-    for (int n = 0; n < kNumSpec; ++n) y[n] = xn[n];
-    y[kNumSpec] = e0;
-    ThetaNet net{a.M, a.ws + c, a.cap, a.htape, a.hoff[static_cast<size_t>(step) * a.cap + c],
-                 a.hcap, a.cap, c, false};
-    Counters cnt;
-    const int result = integrate_theta(y, dt_grid, ctl->integ_max_steps, cnt, net);
-    a.hoff[static_cast<size_t>(step + 1) * a.cap + c] = net.hpos;
-    if (net.overflow) adj_error(a.err, -101);
-    const int cv[7] = {cnt.n_step, cnt.n_rhs, cnt.n_jac, cnt.n_decomp, cnt.n_solve,
-                       cnt.n_accept, cnt.n_reject};
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-        s.stats[k * s.cap + c] += static_cast<unsigned long long>(cv[k] > 0 ? cv[k] : 0);
-    if (result != kSuccess) {
-        s.e[c] = e0;
-        set_failure(ctl, result, c);
-        return;
-    }
-#pragma unroll
-    for (int n = 0; n < kNumSpec; ++n) xn[n] = y[n];
-    const double e = y[kNumSpec];
-    const double T = post_step(xn, rho, e);
-#pragma unroll
-    for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
-    s.e[c] = e;
-    s.T[c] = T;
-    s.time[c] = ctl->next_grid_time;
-    s.completed[c] = completed + 1;
-    atomicAdd(&ctl->integrated, 1);
+// dt_grid of the step being recorded
+__global__ void k_adj_dts(const Ctl* ctl, double* dts, int step) {
+    if (!ctl->done) dts[step] = ctl->dt_grid;
 }
 
 // ---- one accepted ROS2S step at a fixed h -------------------------------------------------
@@ -281,75 +210,80 @@ __device__ __forceinline__ void lu_solve_t_ws(const double* __restrict__ LU, int
     }
 }
 
-// adjoint of f_theta at u (workspace slot) with seed rb: yb += u_bar, Mbar += M_bar
-__device__ __forceinline__ void adj_f(const double* u, int ust, const double (&rb)[kNeqs],
-                                      const double* M, double* w, int st, double (&ub)[kNeqs]) {
-    theta_f_vjp(u, ust, rb, M, w, st);
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) ub[i] = w[(kWV + kVUB + i) * st];
-    for (int r = 0; r < kR; ++r) w[(kWMB + r) * st] += w[(kWMT + r) * st];
-}
+// Stored pieces of one accepted step (kept by k_adj_steps for the reverse
+// sweep): y at its start, the LU factors and pivots of E, k1..k3, u2, u3.
+constexpr int kSY = 0, kSLU = 15, kSK1 = 240, kSK2 = 255, kSK3 = 270, kSU2 = 285, kSU3 = 300,
+              kSPiv = 315, kSub = 316;
 
-// adjoint of one accepted step (after adj_substep at the same y): yb in/out
+// Private memory of a reverse-sweep thread (the hardware interleaves it by lane):
+// W, the y-adjoint output of the generated code, dL/dM of one f adjoint and the
+// cell's accumulated dL/dM, and the staging slots.
+constexpr int kPW = 0, kPUB = 225, kPMT = 240, kPMB = 240 + kR, kPSTG = 240 + 2 * kR;
+constexpr int kPrivRev = kPSTG + AD_NSLOT;
+
+// adjoint of one accepted step: yb in/out.  LU, k, u from the stored slot
+// (element e at slot[e * sst]); W, staging and dL/dM in private memory p.
+// Stages 3, 2, 1 run through one loop body, so each generated adjoint is
+// instantiated once.
 __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], double h,
-                                                   const double* M, double* w, int st,
-                                                   uint64_t piv, double (&yb)[kNeqs]) {
+                                                   const double* M, const double* slot, int sst,
+                                                   double* p, uint64_t piv, double (&yb)[kNeqs]) {
     using C = Ros2sCoef;
-    const double* LU = w + kWLU * st;
-    double* W = w + kWW * st;
-    double kb1[kNeqs], kb2[kNeqs], rb[kNeqs], ub[kNeqs];
+    const double* LU = slot + kSLU * sst;
+    double kb1[kNeqs], kb2[kNeqs], kb3[kNeqs];
 #pragma unroll
     for (int i = 0; i < kNeqs; ++i) {
         kb1[i] = C::b1 * yb[i];
         kb2[i] = C::b2 * yb[i];
-        rb[i] = C::b3 * yb[i];
+        kb3[i] = C::b3 * yb[i];
     }
-    // stage 3
-    lu_solve_t_ws(LU, st, piv, rb);
-    for (int i = 0; i < kNeqs; ++i)
+#pragma unroll 1
+    for (int stage = 3; stage >= 1; --stage) {
+        double rb[kNeqs], u[kNeqs];
+        const int kf = stage == 3 ? kSK3 : stage == 2 ? kSK2 : kSK1;
+        const int uf = stage == 3 ? kSU3 : kSU2;
 #pragma unroll
-        for (int j = 0; j < kNeqs; ++j) W[(i * kNeqs + j) * st] = rb[i] * w[(kWV + kVK3 + j) * st];
+        for (int i = 0; i < kNeqs; ++i) {
+            rb[i] = stage == 3 ? kb3[i] : stage == 2 ? kb2[i] : kb1[i];
+            u[i] = stage == 1 ? y[i] : slot[(uf + i) * sst];
+        }
+        lu_solve_t_ws(LU, sst, piv, rb);
+        if (stage == 3) {
+            for (int i = 0; i < kNeqs; ++i)
 #pragma unroll
-    for (int i = 0; i < kNeqs; ++i) {
-        kb1[i] += (C::c31 / h) * rb[i];
-        kb2[i] += (C::c32 / h) * rb[i];
+                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] = rb[i] * slot[(kf + j) * sst];
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) {
+                kb1[i] += (C::c31 / h) * rb[i];
+                kb2[i] += (C::c32 / h) * rb[i];
+            }
+        } else {
+            for (int i = 0; i < kNeqs; ++i)
+#pragma unroll
+                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] += rb[i] * slot[(kf + j) * sst];
+            if (stage == 2) {
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) kb1[i] += (C::c21 / h) * rb[i];
+            }
+        }
+        ad_f_vjp_st(u, 1, M, 1, rb, 1, p + kPSTG, 1, p + kPUB, p + kPMT);
+        for (int r = 0; r < kR; ++r) p[kPMB + r] += p[kPMT + r];
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) {
+            const double ub = p[kPUB + i];
+            yb[i] += ub;
+            if (stage == 3) {
+                kb1[i] += C::a31 * ub;
+                kb2[i] += C::a32 * ub;
+            } else if (stage == 2) {
+                kb1[i] += C::a21 * ub;
+            }
+        }
     }
-    adj_f(w + (kWV + kVU3) * st, st, rb, M, w, st, ub);
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) {
-        yb[i] += ub[i];
-        kb1[i] += C::a31 * ub[i];
-        kb2[i] += C::a32 * ub[i];
-    }
-    // stage 2
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) rb[i] = kb2[i];
-    lu_solve_t_ws(LU, st, piv, rb);
-    for (int i = 0; i < kNeqs; ++i)
-#pragma unroll
-        for (int j = 0; j < kNeqs; ++j) W[(i * kNeqs + j) * st] += rb[i] * w[(kWV + kVK2 + j) * st];
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) kb1[i] += (C::c21 / h) * rb[i];
-    adj_f(w + (kWV + kVU2) * st, st, rb, M, w, st, ub);
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) {
-        yb[i] += ub[i];
-        kb1[i] += C::a21 * ub[i];
-    }
-    // stage 1
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) rb[i] = kb1[i];
-    lu_solve_t_ws(LU, st, piv, rb);
-    for (int i = 0; i < kNeqs; ++i)
-#pragma unroll
-        for (int j = 0; j < kNeqs; ++j) W[(i * kNeqs + j) * st] += rb[i] * w[(kWV + kVK1 + j) * st];
-    adj_f(y, 1, rb, M, w, st, ub);
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) yb[i] += ub[i];
     // J(y)
-    theta_jac_vjp(y, w, st);
+    ad_jac_vjp_st(y, 1, p + kPW, 1, p + kPSTG, 1, p + kPUB, nullptr);
 #pragma unroll
-    for (int i = 0; i < kNeqs; ++i) yb[i] += w[(kWV + kVUB + i) * st];
+    for (int i = 0; i < kNeqs; ++i) yb[i] += p[kPUB + i];
 }
 
 // ---- the frozen map forward ------------------------------------------------------------
@@ -389,60 +323,101 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_replay(AdjDev a, const double
 }
 
 // ---- the adjoint -------------------------------------------------------------------------
-__global__ __launch_bounds__(kAdjBlock) void k_adj_backward(AdjDev a, const double* __restrict__ gN,
-                                                            double* __restrict__ g0,
-                                                            double* __restrict__ mbar, int n) {
+// Phase A, one thread per (cell, step) of a window of steps [s0, s1): recompute the
+// step from its checkpoint and store each accepted substep's pieces in slot
+// j = k - hoff[s0] of the cell (sub[(j * kSub + field) * cap + cell]).  The
+// thread's workspace column (same layout as the per-cell one) has stride nA.
+__global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __restrict__ wA,
+                                                         int nA, double* __restrict__ sub, int n,
+                                                         int s0, int s1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * (s1 - s0)) return;
+    const int c = t % n;
+    const int step = s0 + t / n;
+    double* w = wA + t;
+    double sv[kNF];
+    const double* ck = a.ckpt + static_cast<size_t>(step) * kNF * a.cap + c;
+#pragma unroll
+    for (int f = 0; f < kNF; ++f) sv[f] = ck[f * a.cap];
+    const double pin[4] = {sv[14], sv[16], sv[17], a.dts[step]};
+    double y[kNeqs], s1v[2];
+    ad_pre(sv, pin, y, s1v);
+    const int base = a.hoff[static_cast<size_t>(s0) * a.cap + c];
+    const int k0 = a.hoff[static_cast<size_t>(step) * a.cap + c];
+    const int k1 = a.hoff[static_cast<size_t>(step + 1) * a.cap + c];
+    for (int k = k0; k < k1; ++k) {
+        double* slot = sub + static_cast<size_t>(k - base) * kSub * a.cap + c;
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) slot[(kSY + i) * a.cap] = y[i];
+        uint64_t piv;
+        if (adj_substep(y, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, w, nA, piv) != kSuccess)
+            adj_error(a.err, kLuError);
+        for (int e = 0; e < kNeqs * kNeqs; ++e) slot[(kSLU + e) * a.cap] = w[(kWLU + e) * nA];
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) {
+            slot[(kSK1 + i) * a.cap] = w[(kWV + kVK1 + i) * nA];
+            slot[(kSK2 + i) * a.cap] = w[(kWV + kVK2 + i) * nA];
+            slot[(kSK3 + i) * a.cap] = w[(kWV + kVK3 + i) * nA];
+            slot[(kSU2 + i) * a.cap] = w[(kWV + kVU2 + i) * nA];
+            slot[(kSU3 + i) * a.cap] = w[(kWV + kVU3 + i) * nA];
+        }
+        slot[kSPiv * a.cap] = __longlong_as_double(static_cast<long long>(piv));
+    }
+}
+
+// Phase B, one thread per cell: the window's steps in reverse from the stored
+// substeps.  lam [18][cap] carries dL/d(state) across windows; dL/dM
+// accumulates in the workspace (cleared when `first`).
+__global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const double* __restrict__ sub,
+                                                           double* __restrict__ lamv, int n, int s0,
+                                                           int s1, int first, int zero) {
+    using C = Ros2sCoef;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     double* w = a.ws + c;
     const int st = a.cap;
+    // `zero` (always 0) keeps the array in private memory rather than registers
+    double priv[kPrivRev];
+    double* const p = priv + zero;
     double lam[kNF];
 #pragma unroll
-    for (int f = 0; f < kNF; ++f) lam[f] = gN[static_cast<size_t>(f) * a.cap + c];
-    for (int r = 0; r < kR; ++r) w[(kWMB + r) * st] = 0.0;
-    for (int step = a.nsteps - 1; step >= 0; --step) {
+    for (int f = 0; f < kNF; ++f) lam[f] = lamv[static_cast<size_t>(f) * a.cap + c];
+    for (int r = 0; r < kR; ++r) p[kPMB + r] = first ? 0.0 : w[(kWMB + r) * st];
+    const int base = a.hoff[static_cast<size_t>(s0) * a.cap + c];
+    for (int step = s1 - 1; step >= s0; --step) {
         double sv[kNF];
         const double* ck = a.ckpt + static_cast<size_t>(step) * kNF * a.cap + c;
 #pragma unroll
         for (int f = 0; f < kNF; ++f) sv[f] = ck[f * a.cap];
         const double pin[4] = {sv[14], sv[16], sv[17], a.dts[step]};
-        double y[kNeqs], s1[2];
-        ad_pre(sv, pin, y, s1);
+        double y[kNeqs], s1v[2];
+        ad_pre(sv, pin, y, s1v);
         const int k0 = a.hoff[static_cast<size_t>(step) * a.cap + c];
         const int k1 = a.hoff[static_cast<size_t>(step + 1) * a.cap + c];
-        const int nk = k1 - k0;
-        if (nk + 1 > a.ys_cap) {
-            adj_error(a.err, -101);
-            return;
-        }
-        double* ys = w + kWYS * st;
+        if (k1 > k0) {
+            // y after the step's last substep, from its stored pieces
+            const double* slot = sub + static_cast<size_t>(k1 - 1 - base) * kSub * a.cap + c;
 #pragma unroll
-        for (int i = 0; i < kNeqs; ++i) ys[i * st] = y[i];
-        for (int k = 0; k < nk; ++k) {
-            uint64_t piv;
-            adj_substep(y, a.htape[static_cast<size_t>(k0 + k) * a.cap + c], a.M, w, st, piv);
-#pragma unroll
-            for (int i = 0; i < kNeqs; ++i) ys[((k + 1) * kNeqs + i) * st] = y[i];
+            for (int i = 0; i < kNeqs; ++i)
+                y[i] = slot[(kSY + i) * a.cap] + C::b1 * slot[(kSK1 + i) * a.cap] +
+                       C::b2 * slot[(kSK2 + i) * a.cap] + C::b3 * slot[(kSK3 + i) * a.cap];
         }
         // post-step sequence
-        const double ps[4] = {0.0, s1[0], 0.0, 0.0};
+        const double ps[4] = {0.0, s1v[0], 0.0, 0.0};
         const double wT[1] = {lam[14]};
         double yb[kNeqs], rb[1];
         ad_post_vjp(y, ps, lam, wT, yb, rb);
         yb[kNumSpec] += lam[15];
         const double rho1b = lam[16] + rb[0];
-        // substeps in reverse, each recomputed from its stored y
-        for (int k = nk - 1; k >= 0; --k) {
-            double yk[kNeqs], yt[kNeqs];
+        for (int k = k1 - 1; k >= k0; --k) {
+            const double* slot = sub + static_cast<size_t>(k - base) * kSub * a.cap + c;
+            double yk[kNeqs];
 #pragma unroll
-            for (int i = 0; i < kNeqs; ++i) {
-                yk[i] = ys[(k * kNeqs + i) * st];
-                yt[i] = yk[i];
-            }
-            const double h = a.htape[static_cast<size_t>(k0 + k) * a.cap + c];
-            uint64_t piv;
-            adj_substep(yt, h, a.M, w, st, piv);
-            adj_substep_reverse(yk, h, a.M, w, st, piv, yb);
+            for (int i = 0; i < kNeqs; ++i) yk[i] = slot[(kSY + i) * a.cap];
+            const uint64_t piv =
+                static_cast<uint64_t>(__double_as_longlong(slot[kSPiv * a.cap]));
+            adj_substep_reverse(yk, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, slot, a.cap,
+                                p, piv, yb);
         }
         // density update and eos_rt
         const double ws2[2] = {rho1b, lam[17]};
@@ -456,13 +431,24 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_backward(AdjDev a, const doub
         lam[17] = sb[2];
     }
 #pragma unroll
-    for (int f = 0; f < kNF; ++f) g0[static_cast<size_t>(f) * a.cap + c] = lam[f];
-    for (int r = 0; r < kR; ++r) mbar[static_cast<size_t>(r) * a.cap + c] = w[(kWMB + r) * st];
+    for (int f = 0; f < kNF; ++f) lamv[static_cast<size_t>(f) * a.cap + c] = lam[f];
+    for (int r = 0; r < kR; ++r) w[(kWMB + r) * st] = p[kPMB + r];
+}
+
+__global__ void k_adj_mbar(AdjDev a, double* __restrict__ mbar, int n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int r = 0; r < kR; ++r) mbar[static_cast<size_t>(r) * a.cap + c] = a.ws[(kWMB + r) * a.cap + c];
 }
 
 // ---- host state ------------------------------------------------------------------------------
 struct AdjointState {
-    int nsteps = 0, hcap = 0, ys_cap = 0, cap = 0, n = 0;
+    int nsteps = 0, hcap = 0, cap = 0, n = 0;
+    std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
+    double* sub = nullptr;    // stored substeps of one window of steps
+    size_t sub_elems = 0;
+    double* wA = nullptr;     // workspace of the step-recompute threads
+    size_t wA_elems = 0;
     bool recorded = false;
     double* ckpt = nullptr;
     int* hoff = nullptr;
@@ -493,6 +479,8 @@ void wr::detail::adjoint_destroy(wr_context* ctx) {
     (void)hipFree(a->M);
     (void)hipFree(a->Mr);
     (void)hipFree(a->ws);
+    (void)hipFree(a->sub);
+    (void)hipFree(a->wA);
     (void)hipFree(a->err);
     (void)hipFree(a->io);
     if (a->ev0) (void)hipEventDestroy(a->ev0);
@@ -515,21 +503,24 @@ AdjDev dev_view(const AdjointState* a, const double* M) {
     d.hcap = a->hcap;
     d.nsteps = a->nsteps;
     d.cap = a->cap;
-    d.ys_cap = a->ys_cap;
     return d;
 }
 
-int ensure_ws(wr_context* ctx, AdjointState* a, int ys_cap) {
-    const size_t need = static_cast<size_t>(kWsFixed + ys_cap * kNeqs) * a->cap;
-    if (a->ws && a->ws_elems >= need) {
-        a->ys_cap = static_cast<int>((a->ws_elems / a->cap - kWsFixed) / kNeqs);
-        return WR_SUCCESS;
-    }
-    (void)hipFree(a->ws);
-    a->ws = nullptr;
+int ensure_ws(wr_context* ctx, AdjointState* a) {
+    if (a->ws) return WR_SUCCESS;
+    const size_t need = static_cast<size_t>(kWsFixed) * a->cap;
     WR_HIP(hipMalloc(&a->ws, need * sizeof(double)), "hipMalloc(adjoint workspace)");
     a->ws_elems = need;
-    a->ys_cap = ys_cap;
+    return WR_SUCCESS;
+}
+
+int ensure_buf(double** p, size_t* have, size_t need, const char* what) {
+    if (*p && *have >= need) return WR_SUCCESS;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    WR_HIP(hipMalloc(p, need * sizeof(double)), what);
+    *have = need;
     return WR_SUCCESS;
 }
 
@@ -600,7 +591,7 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     }
     a->nsteps = nsteps;
     a->n = ctx->n;
-    int st = ensure_ws(ctx, a, 1);
+    int st = ensure_ws(ctx, a);
     if (st != WR_SUCCESS) return st;
     WR_HIP(hipMemcpyAsync(a->M, multipliers, kR * sizeof(double), hipMemcpyHostToDevice,
                           ctx->stream),
@@ -609,13 +600,25 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     WR_HIP(hipMemsetAsync(a->hoff, 0, static_cast<size_t>(a->cap) * sizeof(int), ctx->stream),
            "hipMemset");
     if ((st = ctl_init(ctx, 0, nsteps, 0, 0.0)) != WR_SUCCESS) return st;
-    const AdjDev d = dev_view(a, a->M);
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
+    // the production queue-fed kernel on f_theta, with the tape hooks
+    int cus = 0, per_cu = 0;
+    WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+           "hipDeviceGetAttribute(CU count)");
+    WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_advance_q<false, QThetaNet>,
+                                                        kQBlock, 0),
+           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    int qblocks = cus * (per_cu < 1 ? 1 : per_cu);
+    if (qblocks > blocks_for(ctx->cap, kQBlock)) qblocks = blocks_for(ctx->cap, kQBlock);
+    if (qblocks < 1) qblocks = 1;
     for (int step = 0; step < nsteps; ++step) {
         if ((st = ctl_prepare(ctx)) != WR_SUCCESS) return st;
-        hipLaunchKernelGGL(k_adj_record, dim3(blocks_for(ctx->n, kAdjBlock)), dim3(kAdjBlock), 0,
-                           ctx->stream, ctx->cells, ctx->ctl, d, step);
-        WR_HIP(hipGetLastError(), "k_adj_record launch");
+        if ((st = ctl_queue_order(ctx)) != WR_SUCCESS) return st;
+        QThetaNet net{a->M, a->ckpt, a->hoff, a->htape, a->err, a->hcap, a->cap, step, 0};
+        hipLaunchKernelGGL((k_advance_q<false, QThetaNet>), dim3(qblocks), dim3(kQBlock), 0,
+                           ctx->stream, ctx->cells, ctx->ctl, 0, net);
+        WR_HIP(hipGetLastError(), "k_advance_q (recording) launch");
+        hipLaunchKernelGGL(k_adj_dts, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl, a->dts, step);
         if ((st = ctl_commit(ctx)) != WR_SUCCESS) return st;
     }
     if ((st = timed_end(ctx, a)) != WR_SUCCESS) return st;
@@ -629,18 +632,10 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     if (code != 0 || ctx->ctl_host->completed != nsteps)
         return fail(WR_ERR_ARGUMENT,
                     "wr_adjoint_record: every cell must integrate in every outer step");
-    // largest number of accepted steps of a cell in one outer step
-    std::vector<int> hoff(static_cast<size_t>(nsteps + 1) * a->cap);
-    WR_HIP(hipMemcpy(hoff.data(), a->hoff, hoff.size() * sizeof(int), hipMemcpyDeviceToHost),
+    a->hoff_h.assign(static_cast<size_t>(nsteps + 1) * a->cap, 0);
+    WR_HIP(hipMemcpy(a->hoff_h.data(), a->hoff, a->hoff_h.size() * sizeof(int),
+                     hipMemcpyDeviceToHost),
            "hipMemcpy(adjoint tape index)");
-    int most = 0;
-    for (int s = 0; s < nsteps; ++s)
-        for (int c = 0; c < ctx->n; ++c) {
-            const int k = hoff[static_cast<size_t>(s + 1) * a->cap + c] -
-                          hoff[static_cast<size_t>(s) * a->cap + c];
-            if (k > most) most = k;
-        }
-    if ((st = ensure_ws(ctx, a, most + 1)) != WR_SUCCESS) return st;
     a->recorded = true;
     return kSuccess;
 }
@@ -723,28 +718,67 @@ int wr_adjoint_replay(wr_context* ctx, const double* s0, const double* multiplie
     return code == 0 ? kSuccess : kLuError;
 }
 
+// Memory for the stored substeps and the step-recompute workspace of one
+// window of steps; windows are taken from the last step backwards.
+constexpr size_t kWindowBudgetBytes = size_t(24) << 30;
+
 int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* mbar) {
     if (!ctx || !ctx->adj || !ctx->adj->recorded || !gN || !g0 || !mbar)
         return fail(WR_ERR_ARGUMENT, "wr_adjoint_backward: nothing recorded or bad argument");
     AdjointState* a = ctx->adj;
     WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     const size_t cap = static_cast<size_t>(a->cap);
-    double* d_gN = a->io + 2 * kNF * cap;
-    double* d_g0 = a->io;
+    const int n = a->n;
+    double* d_lam = a->io + 2 * kNF * cap;
     double* d_mb = a->io + 3 * kNF * cap;
     int st;
-    if ((st = put_fields(ctx, d_gN, gN, kNF)) != WR_SUCCESS) return st;
+    if ((st = put_fields(ctx, d_lam, gN, kNF)) != WR_SUCCESS) return st;
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
+    const AdjDev d = dev_view(a, a->M);
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
-    hipLaunchKernelGGL(k_adj_backward, dim3(blocks_for(a->n, kAdjBlock)), dim3(kAdjBlock), 0,
-                       ctx->stream, dev_view(a, a->M), d_gN, d_g0, d_mb, a->n);
-    WR_HIP(hipGetLastError(), "k_adj_backward launch");
+    int s1 = a->nsteps;
+    while (s1 > 0) {
+        // the longest window ending at s1 that fits the budget (at least one step)
+        int s0 = s1 - 1;
+        auto bytes = [&](int lo) {
+            int most = 0;
+            for (int c = 0; c < n; ++c) {
+                const int k = a->hoff_h[static_cast<size_t>(s1) * cap + c] -
+                              a->hoff_h[static_cast<size_t>(lo) * cap + c];
+                if (k > most) most = k;
+            }
+            return (static_cast<size_t>(most) * kSub * cap +
+                    static_cast<size_t>(s1 - lo) * n * kWsSteps) * sizeof(double);
+        };
+        while (s0 > 0 && bytes(s0 - 1) <= kWindowBudgetBytes) --s0;
+        int most = 1;
+        for (int c = 0; c < n; ++c) {
+            const int k = a->hoff_h[static_cast<size_t>(s1) * cap + c] -
+                          a->hoff_h[static_cast<size_t>(s0) * cap + c];
+            if (k > most) most = k;
+        }
+        const int nA = n * (s1 - s0);
+        if ((st = ensure_buf(&a->sub, &a->sub_elems, static_cast<size_t>(most) * kSub * cap,
+                             "hipMalloc(adjoint substeps)")) != WR_SUCCESS ||
+            (st = ensure_buf(&a->wA, &a->wA_elems, static_cast<size_t>(nA) * kWsSteps,
+                             "hipMalloc(adjoint step workspace)")) != WR_SUCCESS)
+            return st;
+        hipLaunchKernelGGL(k_adj_steps, dim3(blocks_for(nA, kAdjBlock)), dim3(kAdjBlock), 0,
+                           ctx->stream, d, a->wA, nA, a->sub, n, s0, s1);
+        WR_HIP(hipGetLastError(), "k_adj_steps launch");
+        hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, kAdjBlock)), dim3(kAdjBlock), 0,
+                           ctx->stream, d, a->sub, d_lam, n, s0, s1, s1 == a->nsteps ? 1 : 0, 0);
+        WR_HIP(hipGetLastError(), "k_adj_reverse launch");
+        s1 = s0;
+    }
+    hipLaunchKernelGGL(k_adj_mbar, dim3(blocks_for(n, kAdjBlock)), dim3(kAdjBlock), 0, ctx->stream,
+                       d, d_mb, n);
+    WR_HIP(hipGetLastError(), "k_adj_mbar launch");
     if ((st = timed_end(ctx, a)) != WR_SUCCESS) return st;
-    if ((st = get_fields(ctx, g0, d_g0, kNF)) != WR_SUCCESS) return st;
+    if ((st = get_fields(ctx, g0, d_lam, kNF)) != WR_SUCCESS) return st;
     if ((st = get_fields(ctx, mbar, d_mb, kR)) != WR_SUCCESS) return st;
     int code = 0;
     if ((st = read_err(ctx, a, &code)) != WR_SUCCESS) return st;
-    if (code == -101) return fail(WR_ERR_CAPACITY, "wr_adjoint_backward: substep workspace full");
     return code == 0 ? kSuccess : kLuError;
 }
 

@@ -146,6 +146,7 @@ namespace detail {
 int ctl_init(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time);
 int ctl_read(wr_context* ctx);
 int ctl_prepare(wr_context* ctx);  // k_prepare + k_plan of one outer step
+int ctl_queue_order(wr_context* ctx);  // queue order of the queue-fed advance
 int ctl_commit(wr_context* ctx);   // k_commit of one outer step
 void ctl_fill_info(wr_context* ctx);
 void adjoint_destroy(wr_context* ctx);  // wr_adjoint.hip

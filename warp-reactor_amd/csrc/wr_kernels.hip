@@ -17,6 +17,7 @@
 // run is done, so the host enqueues several steps ahead and polls one flag.
 #include "wr_ctx.hpp"
 #include "wr_device.hpp"
+#include "wr_queue.hpp"
 
 #include <float.h>
 #include <stdio.h>
@@ -333,310 +334,18 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(DevCells s, Ctl* 
     if (c < s.n) s.order[base[b] + rank] = c;
 }
 
-// ---- queue-fed advance ---------------------------------------------------------------
-// The same per-cell work as k_advance (reproducer.cpp:7736-7785 with
-// RODAS::integrate 7205-7347), reorganised for the GPU:
-//  * a fixed grid of persistent threads takes cells from an atomic queue;
-//  * the integrator is a flat state machine -- one loop iteration is one
-//    ROS2S attempt (or one singular-matrix retry) of the cell a lane holds --
-//    so a lane whose cell finishes takes the next cell at the next iteration
-//    instead of idling until its wave's slowest cell is done;
-//  * J, LU and the stage vectors live in a per-thread SoA workspace; the
-//    network code is register-budgeted (staging slots in the same workspace),
-//    so nothing is left to compiler spilling.
-// Counters, results and every arithmetic operation are those of the
-// reference; only where values are kept differs.
-constexpr int kQBlock = 256;
-
-// The per-thread workspace is the lane's private memory: the hardware
-// interleaves private (scratch) memory by lane, so element e of all 64 lanes
-// of a wave is one contiguous 512-byte line, and an element is addressed by
-// an immediate offset.  `zero` (always 0) is an index the compiler cannot
-// resolve, which keeps the array in memory instead of registers.
-constexpr int kPrivLen = 2 * kNeqs * kNeqs + kVecLen + kStageSlots;
-
-// The first kSoloLanes queue positions -- the cells that took the most
-// attempts last time -- go to the first wave of block 0, which runs alone on
-// its CU (the block's other waves leave at once): the one cell that needs
-// tens of attempts then does not share instruction issue and fetch with
-// three other waves while the bulk of the cells is being advanced.
-constexpr int kSoloLanes = 64;
-
-// kMixed: the FP32-factorisation form (E = fac I - J factorised in FP32, FP64
-// residual and refinement sweeps per stage solve, FP64 factors when the FP32
-// ones are singular or a refinement does not converge); the RHS, energy
-// equation, error norm and controller stay FP64.
-// Stage solve of the queue-fed kernel: FP64 factors, or (kMixed) FP32 factors
-// with refinement and the FP64 factors from the first non-convergence on.
-template <bool kMixed>
-__device__ __forceinline__ void stage_solve(const ThreadWs& w, const float* lu32, uint64_t piv32,
-                                            const int (&ks)[kNeqs], double fac, int sweeps,
-                                            Ctl* ctl, bool& use64, uint64_t& piv,
-                                            double (&v)[kNeqs]) {
-    if constexpr (kMixed) {
-        if (!use64) {
-            double b[kNeqs];
-#pragma unroll
-            for (int i = 0; i < kNeqs; ++i) b[i] = v[i];
-            if (solve_mixed(w, lu32, piv32, ks, fac, sweeps, v)) return;
-            atomicAdd(&ctl->mixed_refine_fallbacks, 1ull);
-            if (lu_left(w, fac, piv) != 0) return;  // keep the refined FP32 result
-            use64 = true;
-#pragma unroll
-            for (int i = 0; i < kNeqs; ++i) v[i] = b[i];
-        }
-    }
-    lu_solve_ws(w, piv, v);
-}
-
-template <bool kMixed>
-__global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero) {
-    using C = Ros2sCoef;
-    if (ctl->done) return;
-    bool solo = blockIdx.x == 0;
-    if (solo && threadIdx.x >= kSoloLanes) return;
-    double priv[kPrivLen];
-    double* const p = priv + zero;
-    float priv32[kMixed ? kNeqs * kNeqs : 1];
-    float* const lu32 = priv32 + zero;
-    const int sweeps = ctl->lu_sweeps;
-    ThreadWs w;
-    w.J = p;
-    w.LU = p + kNeqs * kNeqs;
-    w.vec = p + 2 * kNeqs * kNeqs;
-    w.stage = p + 2 * kNeqs * kNeqs + kVecLen;
-    w.stride = 1;
-    const int st = 1;
-    const int completed = ctl->completed;
-    const double dt_grid = ctl->dt_grid;
-    const double next_time = ctl->next_grid_time;
-    const int max_steps = ctl->integ_max_steps;
-    int c = -1;
-    double x = 0.0, h = 0.0, hopt = 0.0, hacc = 0.0, erracc = 1.0, rho = 0.0, e0 = 0.0;
-    int nsing = 0, n_step = 0, n_accept = 0;
-    bool reject = false, last = false, top = true;
-    Counters cnt{0, 0, 0, 0, 0, 0, 0};
-    uint64_t piv = 0, piv32 = 0;
-    for (;;) {
-        if (c < 0) {
-            // ---- take the next cell: advance_collapse_gridwide_kernel's preamble
-            int q;
-            if (solo) {
-                q = atomicAdd(&ctl->queue_solo, 1);
-                if (q >= kSoloLanes) {
-                    solo = false;  // reserved positions used up: join the common queue
-                    continue;
-                }
-            } else {
-                q = kSoloLanes + atomicAdd(&ctl->queue, 1);
-            }
-            if (q >= s.n || ctl->failure != kSuccess) break;
-            c = s.order[q];
-            if (s.completed[c] != completed) {
-                c = -1;
-                continue;
-            }
-            double xn[kNumSpec];
-#pragma unroll
-            for (int n = 0; n < kNumSpec; ++n) xn[n] = s.xn[n * s.cap + c];
-            const double local_dt = collapse_timestep(xn);
-            if (local_dt < 10.0) {
-                c = -1;
-                continue;
-            }
-            const double old_density = s.driver[c];
-            const double tff = local_dt / 1.0e-1;
-            const double driver = old_density + dt_grid * (old_density / tff);
-            s.driver[c] = driver;
-            if (!valid_positive(driver) || driver > 2.0e-6) {
-                c = -1;
-                continue;
-            }
-            const double ratio = driver / old_density;
-#pragma unroll
-            for (int n = 0; n < kNumSpec; ++n) {
-                xn[n] *= ratio;
-                s.xn[n * s.cap + c] = xn[n];  // the kernel scales the state in place
-            }
-            rho = s.rho[c] * ratio;
-            s.rho[c] = rho;
-            // burn_ros2s (7592-7606): eos_rt, y = (xn, e)
-            e0 = eos_energy(xn, s.T[c]);
-#pragma unroll
-            for (int n = 0; n < kNumSpec; ++n) WR_AT(w.vec, kVecY + n, st) = xn[n];
-            WR_AT(w.vec, kVecY + kNumSpec, st) = e0;
-            // integrate() prologue, t = 0, tout = dt = dt_grid (> 0, checked by k_plan)
-            h = dt_grid;
-            if (fabs(h) <= 10.0 * kUround) h = 1.0e-6;
-            h = rmin(fabs(h), dt_grid - 0.0);
-            x = 0.0;
-            hopt = h;
-            hacc = 0.0;
-            erracc = 1.0;
-            nsing = 0;
-            n_step = 0;
-            n_accept = 0;
-            reject = false;
-            last = false;
-            top = true;
-            cnt = Counters{0, 0, 0, 0, 0, 0, 0};
-        }
-        int result = 0;  // 0: the cell goes on
-        if (top) {
-            // head of the outer loop of integrate (7236-7258)
-            if (n_step > max_steps) {
-                result = kTooManySteps;
-            } else if (0.1 * fabs(h) <= fabs(x) * kUround) {
-                result = kDtUnderflow;
-            } else if (last) {
-                result = kSuccess;
-            } else {
-                hopt = h;
-                if (x + h * 1.0001 >= dt_grid) {
-                    h = dt_grid - x;
-                    last = true;
-                }
-                ws_jac_rhs(w);
-                cnt.n_jac += 1;
-                top = false;
-            }
-        }
-        if (result == 0) {
-            // retry loop body (7260-7345): factorize, three stages, controller
-            const double fac = 1.0 / (h * C::gamma);
-            bool use64 = !kMixed;
-            int info;
-            int ks[kNeqs];
-            if constexpr (kMixed) {
-                mixed_scales(w, ks);
-                info = lu_left_t<float>(w.J, lu32, st, fac, piv32, ks);
-                if (info != 0) {  // singular in FP32: FP64 factors for this attempt
-                    atomicAdd(&ctl->mixed_lu_fallbacks, 1ull);
-                    info = lu_left(w, fac, piv);
-                    use64 = true;
-                }
-            } else {
-                info = lu_left(w, fac, piv);
-            }
-            if (info != 0) {
-                nsing += 1;
-                if (nsing >= 5) {
-                    result = kLuError;
-                } else {
-                    h *= 0.5;
-                    reject = true;
-                    last = false;
-                }
-            } else {
-                cnt.n_decomp += 1;
-                double v[kNeqs];
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i) v[i] = WR_AT(w.vec, kVecF0 + i, st);
-                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak1
-                cnt.n_solve += 1;
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i) {
-                    WR_AT(w.vec, kVecAk1 + i, st) = v[i];
-                    WR_AT(w.vec, kVecYnew + i, st) = WR_AT(w.vec, kVecY + i, st) + C::a21 * v[i];
-                    WR_AT(w.vec, kVecAk2 + i, st) = (C::c21 / h) * v[i];
-                }
-                ws_rhs(w);
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i)
-                    v[i] = WR_AT(w.vec, kVecAk2 + i, st) + WR_AT(w.vec, kVecDy + i, st);
-                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak2
-                cnt.n_solve += 1;
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i) {
-                    const double a1 = WR_AT(w.vec, kVecAk1 + i, st);
-                    WR_AT(w.vec, kVecAk2 + i, st) = v[i];
-                    WR_AT(w.vec, kVecYnew + i, st) =
-                        WR_AT(w.vec, kVecY + i, st) + C::a31 * a1 + C::a32 * v[i];
-                    WR_AT(w.vec, kVecAk3 + i, st) = (C::c31 * a1 + C::c32 * v[i]) / h;
-                }
-                ws_rhs(w);
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i)
-                    v[i] = WR_AT(w.vec, kVecAk3 + i, st) + WR_AT(w.vec, kVecDy + i, st);
-                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak3
-                cnt.n_solve += 1;
-                double err = 0.0;
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i) {
-                    const double a1 = WR_AT(w.vec, kVecAk1 + i, st);
-                    const double a2 = WR_AT(w.vec, kVecAk2 + i, st);
-                    const double yi = WR_AT(w.vec, kVecY + i, st);
-                    const double yn = yi + C::b1 * a1 + C::b2 * a2 + C::b3 * v[i];
-                    WR_AT(w.vec, kVecYnew + i, st) = yn;
-                    const double wv = C::e1 * a1 + C::e2 * a2 + C::e3 * v[i];
-                    const double sk = atol_of(i) + rtol_of(i) * rmax(fabs(yi), fabs(yn));
-                    const double term = wv / sk;
-                    err += term * term;
-                }
-                cnt.n_rhs += 3;
-                n_step += 1;
-                cnt.n_step = n_step;
-                err = sqrt(err / static_cast<double>(kNeqs));
-                const double raw_fac = WR_CBRT(err) / kSafe;
-                const double fac_step = rmax(1.0 / kFacMax, rmin(1.0 / kFacMin, raw_fac));
-                double hnew = h / fac_step;
-                if (err <= 1.0) {
-                    n_accept += 1;
-                    cnt.n_accept = n_accept;
-                    if (n_accept > 1) {
-                        const double facgus = rmax(
-                            1.0 / kFacMax,
-                            rmin(1.0 / kFacMin, (hacc / h) * WR_CBRT((err * err) / erracc) / kSafe));
-                        hnew = h / rmax(fac_step, facgus);
-                    }
-                    hacc = h;
-                    erracc = rmax(1.0e-2, err);
-#pragma unroll
-                    for (int i = 0; i < kNeqs; ++i)
-                        WR_AT(w.vec, kVecY + i, st) = WR_AT(w.vec, kVecYnew + i, st);
-                    x += h;
-                    if (fabs(hnew) > dt_grid) hnew = dt_grid;
-                    if (reject) hnew = rmin(fabs(hnew), fabs(h));
-                    reject = false;
-                    h = hnew;
-                    top = true;
-                } else {
-                    reject = true;
-                    last = false;
-                    h = hnew;
-                    if (n_accept >= 1) cnt.n_reject += 1;
-                }
-            }
-        }
-        if (result != 0) {
-            // ---- the cell is done: burn_ros2s epilogue and the kernel's commit
-            const int cv[7] = {cnt.n_step, cnt.n_rhs, cnt.n_jac, cnt.n_decomp, cnt.n_solve,
-                               cnt.n_accept, cnt.n_reject};
-#pragma unroll
-            for (int k = 0; k < 7; ++k)
-                s.stats[k * s.cap + c] += static_cast<unsigned long long>(cv[k] > 0 ? cv[k] : 0);
-            s.cost[c] = cnt.n_step;
-            if (result == kSuccess) {
-                double xn[kNumSpec];
-#pragma unroll
-                for (int n = 0; n < kNumSpec; ++n) xn[n] = WR_AT(w.vec, kVecY + n, st);
-                const double e = WR_AT(w.vec, kVecY + kNumSpec, st);
-                const double T = post_step(xn, rho, e);
-#pragma unroll
-                for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
-                s.e[c] = e;
-                s.T[c] = T;
-                s.time[c] = next_time;
-                s.completed[c] = completed + 1;
-                atomicAdd(&ctl->integrated, 1);
-            } else {
-                s.e[c] = e0;  // eos_rt ran before the failed burn
-                set_failure(ctl, result, c);
-            }
-            c = -1;
-        }
-    }
-}
+// ---- queue-fed advance: wr_queue.hpp -------------------------------------------------
+// The production network: the generated staged J + f and f, no tape.
+struct QPlainNet {
+    static constexpr int kSlots = kStageSlots;
+    __device__ __forceinline__ void jac_rhs(const ThreadWs& w) { ws_jac_rhs(w); }
+    __device__ __forceinline__ void rhs(const ThreadWs& w) { ws_rhs(w); }
+    __device__ __forceinline__ void begin(int, const double (&)[kNumSpec], double, double, double,
+                                          double) {}
+    __device__ __forceinline__ void skipped(int) {}
+    __device__ __forceinline__ void accepted(int, double) {}
+    __device__ __forceinline__ void end(int) {}
+};
 
 __global__ void k_commit(Ctl* ctl) {
     if (ctl->done) return;
@@ -992,7 +701,8 @@ static int queue_grid(wr_context* ctx) {
     WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
            "hipDeviceGetAttribute(CU count)");
     WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-               &per_cu, ctx->lu_mode ? k_advance_q<true> : k_advance_q<false>, kQBlock, 0),
+               &per_cu, ctx->lu_mode ? k_advance_q<true, QPlainNet> : k_advance_q<false, QPlainNet>,
+               kQBlock, 0),
            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (per_cu < 1) per_cu = 1;
     int blocks = cus * per_cu;
@@ -1015,11 +725,11 @@ static int launch_advance(wr_context* ctx) {
         hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
                            ctx->ctl);
         if (ctx->lu_mode)
-            hipLaunchKernelGGL(k_advance_q<true>, dim3(ctx->q_blocks), dim3(kQBlock), 0,
-                               ctx->stream, ctx->cells, ctx->ctl, 0);
+            hipLaunchKernelGGL((k_advance_q<true, QPlainNet>), dim3(ctx->q_blocks), dim3(kQBlock), 0,
+                               ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
         else
-            hipLaunchKernelGGL(k_advance_q<false>, dim3(ctx->q_blocks), dim3(kQBlock), 0,
-                               ctx->stream, ctx->cells, ctx->ctl, 0);
+            hipLaunchKernelGGL((k_advance_q<false, QPlainNet>), dim3(ctx->q_blocks), dim3(kQBlock), 0,
+                               ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
     } else {
         hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
                            ctx->stream, ctx->cells, ctx->ws, ctx->ctl);
@@ -1499,6 +1209,15 @@ int ctl_init(wr_context* ctx, int perturb, int max_steps, int completed, double 
 }
 int ctl_read(wr_context* ctx) { return ::read_ctl(ctx); }
 int ctl_prepare(wr_context* ctx) { return ::launch_prepare(ctx); }
+int ctl_queue_order(wr_context* ctx) {
+    const int nb = blocks_for(ctx->n, kOrderBlock);
+    hipLaunchKernelGGL(k_order_count, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
+                       ctx->ctl);
+    hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
+                       ctx->ctl);
+    WR_HIP(hipGetLastError(), "queue order launch");
+    return WR_SUCCESS;
+}
 int ctl_commit(wr_context* ctx) {
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
     WR_HIP(hipGetLastError(), "commit launch");

@@ -530,21 +530,23 @@ def build(out_path, info_path=None, dev_path=None):
         counts[name] = ops
     info["ops"] = counts
     staged = []
-    for name, params, outs, regs in [
+    # register budgets (doubles) and reload distances (operations, extra registers)
+    for name, params, outs, regs, pf, extra in [
             ("ad_fj_st", "AD_STAGE_IN(y), AD_STAGE_IN(M) AD_STAGE_PARAMS",
              [(f"F, {i}", m.f[i]) for i in range(NEQ)] +
-             [(f"J, {k}", m.J[k]) for k in range(225)], 128),
+             [(f"J, {k}", m.J[k]) for k in range(225)], 128, 500, 32),
             ("ad_f_st", "AD_STAGE_IN(y), AD_STAGE_IN(M) AD_STAGE_PARAMS",
-             [(f"F, {i}", m.f[i]) for i in range(NEQ)], 96),
+             [(f"F, {i}", m.f[i]) for i in range(NEQ)], 96, 500, 32),
             ("ad_f_vjp_st", "AD_STAGE_IN(y), AD_STAGE_IN(M), AD_STAGE_IN(w) AD_STAGE_PARAMS",
              [(f"Y, {i}", f_a[i]) for i in range(NEQ)] +
-             [(f"MB, {r}", f_a[NEQ + r]) for r in range(R)], 112),
+             [(f"MB, {r}", f_a[NEQ + r]) for r in range(R)], 112, 500, 32),
             ("ad_jac_vjp_st", "AD_STAGE_IN(y), AD_STAGE_IN(W) AD_STAGE_PARAMS",
-             [(f"Y, {i}", J_a[i]) for i in range(NEQ)], 128)]:
-        text, st = emit_staged_fn(d, name, params, outs, regs)
+             [(f"Y, {i}", J_a[i]) for i in range(NEQ)], 128, 500, 32)]:
+        text, st = emit_staged_fn(d, name, params, outs, regs, prefetch=pf, extra=extra)
         staged.append(text)
         info.setdefault("staged", {})[name] = st
     nslot = max(v["slots"] for v in info["staged"].values())
+    fwd_slots = max(info["staged"]["ad_fj_st"]["slots"], info["staged"]["ad_f_st"]["slots"])
     hdr = [
         "/* Generated by warp-reactor_amd/gen/ad.py from primordial_network.ir. Do not edit.",
         " * Forward, tangent-linear and adjoint code of one cell's frozen-step collapse",
@@ -554,6 +556,7 @@ def build(out_path, info_path=None, dev_path=None):
         " * STAGE_STORE/LOAD and AD_OUT. */",
         f"#define AD_NTHETA {R}",
         f"#define AD_NSLOT {max(nslot, 1)}",
+        f"#define AD_FWD_NSLOT {max(fwd_slots, 1)}",
         "",
     ]
     # the CPU header: every plain function; the device header: the staged
