@@ -126,6 +126,8 @@ def main():
                     help="skip the FP32-factorisation run")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (32^3, 20 outer steps)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="enqueue outer steps as individual launches instead of HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -145,6 +147,7 @@ def main():
                 f"T {t0:g}-{t1:g} K) with splitmix64 density perturbations")
     ctx = wr.Context(n)
     ctx.set_advance_kernel(args.advance_kernel)
+    ctx.set_graph(not args.no_graph)
     # prime: load the kernels with one outer step (not timed, not counted)
     ctx.upload(init)
     ctx.run(perturb=True, max_steps=1)
