@@ -128,6 +128,10 @@ def main():
                     help="skip the adjoint measurement (32^3, 20 outer steps)")
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue outer steps as individual launches instead of HIP graphs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape of the queue-fed kernel")
+    ap.add_argument("--tune-steps", type=int, default=100,
+                    help="outer steps of each autotuner candidate run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -151,6 +155,14 @@ def main():
     # prime: load the kernels with one outer step (not timed, not counted)
     ctx.upload(init)
     ctx.run(perturb=True, max_steps=1)
+    tuned = None
+    if args.advance_kernel == 1 and not args.no_autotune:
+        autotune = importlib.import_module("warp-reactor_amd.autotune")
+        chosen, table = autotune.autotune(ctx, init, steps=min(args.tune_steps, args.max_outer))
+        tuned = {"chosen": {"threads": chosen[0], "blocks_per_cu": chosen[1],
+                            "solo_lanes": chosen[2]},
+                 "steps_per_candidate": min(args.tune_steps, args.max_outer), "candidates": table}
+        print(f"# autotune: {chosen} of {len(table)} shapes", file=sys.stderr, flush=True)
 
     def one_run():
         ctx.upload(init)  # inputs resident in HBM before the clock starts
@@ -217,6 +229,8 @@ def main():
                      "flops_per_launch": per_launch_flops},
         "counters": counters,
     }
+    if tuned:
+        line["autotune"] = tuned
     kname = line["roofline"]["kernel"]
     traffic = measured_traffic(kname, args.grid) if args.recipe == "reference" else None
     line["roofline"]["traffic"] = traffic

@@ -167,6 +167,19 @@ int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_tim
  * network code); 0 = one thread per cell.  Results are identical. */
 int wr_set_advance_kernel(wr_context *ctx, int kernel);
 
+/* Launch shape of the queue-fed kernel (no counterpart in the reference,
+ * whose advance is one thread per cell, reproducer.cpp:7736): threads per
+ * block (64..256, multiple of 64), resident blocks per CU (0: as many as the
+ * kernel's occupancy allows), and the queue positions reserved for block 0's
+ * first wave, which then runs alone on its CU (0 or 64; the most expensive
+ * cells of the last step).  Results do not depend on it; the autotuner
+ * (warp-reactor_amd/autotune.py) picks it by timing.  Defaults 256, 0, 64. */
+int wr_set_queue_launch(wr_context *ctx, int threads_per_block, int blocks_per_cu,
+                        int solo_lanes);
+/* The current shape and the grid size last used (0 before the first launch). */
+int wr_get_queue_launch(const wr_context *ctx, int *threads_per_block, int *blocks_per_cu,
+                        int *solo_lanes, int *blocks);
+
 /* Factorisation of I - gamma h J in the queue-fed kernel of subsequent steps
  * (no counterpart in the reference, which factorises in FP64 only:
  * reproducer.cpp:7159-7171).  mode 0 (default): FP64, results identical to the

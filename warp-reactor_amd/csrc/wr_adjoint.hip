@@ -606,16 +606,17 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
            "hipDeviceGetAttribute(CU count)");
     WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_advance_q<false, QThetaNet>,
-                                                        kQBlock, 0),
+                                                        ctx->q_threads, 0),
            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (ctx->q_per_cu > 0) per_cu = ctx->q_per_cu;
     int qblocks = cus * (per_cu < 1 ? 1 : per_cu);
-    if (qblocks > blocks_for(ctx->cap, kQBlock)) qblocks = blocks_for(ctx->cap, kQBlock);
+    if (qblocks > blocks_for(ctx->cap, ctx->q_threads)) qblocks = blocks_for(ctx->cap, ctx->q_threads);
     if (qblocks < 1) qblocks = 1;
     for (int step = 0; step < nsteps; ++step) {
         if ((st = ctl_prepare(ctx)) != WR_SUCCESS) return st;
         if ((st = ctl_queue_order(ctx)) != WR_SUCCESS) return st;
         QThetaNet net{a->M, a->ckpt, a->hoff, a->htape, a->err, a->hcap, a->cap, step, 0};
-        hipLaunchKernelGGL((k_advance_q<false, QThetaNet>), dim3(qblocks), dim3(kQBlock), 0,
+        hipLaunchKernelGGL((k_advance_q<false, QThetaNet>), dim3(qblocks), dim3(ctx->q_threads), 0,
                            ctx->stream, ctx->cells, ctx->ctl, 0, net);
         WR_HIP(hipGetLastError(), "k_advance_q (recording) launch");
         hipLaunchKernelGGL(k_adj_dts, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl, a->dts, step);

@@ -57,6 +57,7 @@ struct Ctl {
     int queue_solo;     // next of the positions reserved for the solo wave
     int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
     int lu_sweeps;        // refinement sweeps per stage solve (FP32-factorisation form)
+    int solo_lanes;       // queue positions reserved for block 0's first wave (0: none)
     unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
     unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
     int bin_count[16];  // cells per attempt-count bin, for the queue order
@@ -125,6 +126,11 @@ struct wr_context {
     int lu_mode = 0;
     int lu_sweeps = 2;
     int integ_max_steps = wr::kMaxStepsDefault;
+    // launch shape of the queue-fed kernel (wr_set_queue_launch): threads per
+    // block, resident blocks per CU (0: the occupancy limit), solo-wave lanes
+    int q_threads = 256;
+    int q_per_cu = 0;
+    int q_solo = 64;
     int q_blocks = 0;
     // HIP-event timing of the advance launches of wr_run (when enabled)
     static constexpr int kChunk = 8;

@@ -665,6 +665,7 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.limiting_cell = -1;
     h.integ_max_steps = ctx->integ_max_steps;
     h.lu_sweeps = ctx->lu_sweeps;
+    h.solo_lanes = ctx->q_solo;
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
            "hipMemcpy(control to device)");
@@ -702,11 +703,12 @@ static int queue_grid(wr_context* ctx) {
            "hipDeviceGetAttribute(CU count)");
     WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                &per_cu, ctx->lu_mode ? k_advance_q<true, QPlainNet> : k_advance_q<false, QPlainNet>,
-               kQBlock, 0),
+               ctx->q_threads, 0),
            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (per_cu < 1) per_cu = 1;
+    if (ctx->q_per_cu > 0) per_cu = ctx->q_per_cu;
     int blocks = cus * per_cu;
-    const int need = blocks_for(ctx->cap, kQBlock);
+    const int need = blocks_for(ctx->cap, ctx->q_threads);
     if (blocks > need) blocks = need;
     ctx->q_blocks = blocks < 1 ? 1 : blocks;
     return WR_SUCCESS;
@@ -725,10 +727,10 @@ static int launch_advance(wr_context* ctx) {
         hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, ctx->stream, ctx->cells,
                            ctx->ctl);
         if (ctx->lu_mode)
-            hipLaunchKernelGGL((k_advance_q<true, QPlainNet>), dim3(ctx->q_blocks), dim3(kQBlock), 0,
+            hipLaunchKernelGGL((k_advance_q<true, QPlainNet>), dim3(ctx->q_blocks), dim3(ctx->q_threads), 0,
                                ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
         else
-            hipLaunchKernelGGL((k_advance_q<false, QPlainNet>), dim3(ctx->q_blocks), dim3(kQBlock), 0,
+            hipLaunchKernelGGL((k_advance_q<false, QPlainNet>), dim3(ctx->q_blocks), dim3(ctx->q_threads), 0,
                                ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
     } else {
         hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
@@ -777,7 +779,8 @@ static int enqueue_steps(wr_context* ctx, int todo, bool timed) {
 // except the cell count and the advance kernel, so the graph is captured
 // again only when those change.  The graph holds kernel nodes only.
 static int chunk_graph(wr_context* ctx) {
-    const int kernel_key = ctx->advance_kernel + 2 * ctx->lu_mode;
+    const int kernel_key = ctx->advance_kernel + 2 * ctx->lu_mode +
+                           4 * (ctx->q_threads + 1024 * (ctx->q_per_cu + 64 * ctx->q_solo));
     if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == kernel_key)
         return WR_SUCCESS;
     if (ctx->chunk_exec) {
@@ -881,6 +884,32 @@ int wr_set_lu_precision(wr_context* ctx, int mode, int sweeps) {
     if (mode != ctx->lu_mode) ctx->q_blocks = 0;  // occupancy of the other kernel
     ctx->lu_mode = mode;
     if (mode == 1) ctx->lu_sweeps = sweeps;
+    return WR_SUCCESS;
+}
+
+int wr_set_queue_launch(wr_context* ctx, int threads_per_block, int blocks_per_cu,
+                        int solo_lanes) {
+    if (!ctx || threads_per_block < 64 || threads_per_block > kQBlock || threads_per_block % 64 ||
+        blocks_per_cu < 0 || blocks_per_cu > 16 || (solo_lanes != 0 && solo_lanes != 64) ||
+        solo_lanes > threads_per_block)
+        return fail(WR_ERR_ARGUMENT,
+                    "wr_set_queue_launch: threads 64..256 in steps of 64, blocks per CU 0..16, "
+                    "solo lanes 0 or 64");
+    ctx->q_threads = threads_per_block;
+    ctx->q_per_cu = blocks_per_cu;
+    ctx->q_solo = solo_lanes;
+    ctx->q_blocks = 0;
+    return WR_SUCCESS;
+}
+
+int wr_get_queue_launch(const wr_context* ctx, int* threads_per_block, int* blocks_per_cu,
+                        int* solo_lanes, int* blocks) {
+    if (!ctx || !threads_per_block || !blocks_per_cu || !solo_lanes || !blocks)
+        return fail(WR_ERR_ARGUMENT, "wr_get_queue_launch: bad argument");
+    *threads_per_block = ctx->q_threads;
+    *blocks_per_cu = ctx->q_per_cu;
+    *solo_lanes = ctx->q_solo;
+    *blocks = ctx->q_blocks;
     return WR_SUCCESS;
 }
 

@@ -38,7 +38,7 @@ constexpr int kQBlock = 256;
 // its CU (the block's other waves leave at once): the one cell that needs
 // tens of attempts then does not share instruction issue and fetch with
 // three other waves while the bulk of the cells is being advanced.
-constexpr int kSoloLanes = 64;
+constexpr int kSoloLanes = 64;  // default of Ctl::solo_lanes
 
 // kMixed: the FP32-factorisation form (E = fac I - J factorised in FP32, FP64
 // residual and refinement sweeps per stage solve, FP64 factors when the FP32
@@ -71,8 +71,9 @@ template <bool kMixed, class Net>
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero, Net net) {
     using C = Ros2sCoef;
     if (ctl->done) return;
-    bool solo = blockIdx.x == 0;
-    if (solo && threadIdx.x >= kSoloLanes) return;
+    const int solo_lanes = ctl->solo_lanes;
+    bool solo = blockIdx.x == 0 && solo_lanes > 0;
+    if (solo && static_cast<int>(threadIdx.x) >= solo_lanes) return;
     double priv[2 * kNeqs * kNeqs + kVecLen + Net::kSlots];
     double* const p = priv + zero;
     float priv32[kMixed ? kNeqs * kNeqs : 1];
@@ -101,12 +102,12 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
             int q;
             if (solo) {
                 q = atomicAdd(&ctl->queue_solo, 1);
-                if (q >= kSoloLanes) {
+                if (q >= solo_lanes) {
                     solo = false;  // reserved positions used up: join the common queue
                     continue;
                 }
             } else {
-                q = kSoloLanes + atomicAdd(&ctl->queue, 1);
+                q = solo_lanes + atomicAdd(&ctl->queue, 1);
             }
             if (q >= s.n || ctl->failure != kSuccess) break;
             c = s.order[q];

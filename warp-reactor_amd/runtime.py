@@ -83,6 +83,9 @@ _SIGS = {
     "wr_set_integrator_max_steps": (_I, [_VP, _I]),
     "wr_set_graph": (_I, [_VP, _I]),
     "wr_set_lu_precision": (_I, [_VP, _I, _I]),
+    "wr_set_queue_launch": (_I, [_VP, _I, _I, _I]),
+    "wr_get_queue_launch": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                 ctypes.POINTER(_I)]),
     "wr_mixed_fallbacks": (_I, [_VP, ctypes.POINTER(ctypes.c_ulonglong),
                                 ctypes.POINTER(ctypes.c_ulonglong)]),
     "wr_last_advance_timing": (_I, [_VP, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
@@ -267,6 +270,18 @@ class Context:
     def set_graph(self, enable=True):
         """Enqueue outer steps as captured HIP graphs (default) or launches."""
         _check(self._lib.wr_set_graph(self._h, int(bool(enable))), "wr_set_graph")
+
+    def set_queue_launch(self, threads_per_block=256, blocks_per_cu=0, solo_lanes=64):
+        """Launch shape of the queue-fed kernel (results do not depend on it)."""
+        _check(self._lib.wr_set_queue_launch(self._h, int(threads_per_block), int(blocks_per_cu),
+                                             int(solo_lanes)), "wr_set_queue_launch")
+
+    def queue_launch(self):
+        """(threads per block, blocks per CU, solo lanes, grid blocks last used)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(self._lib.wr_get_queue_launch(self._h, *[ctypes.byref(x) for x in v]),
+               "wr_get_queue_launch")
+        return tuple(x.value for x in v)
 
     def set_lu_precision(self, mode, sweeps=2):
         """Factorisation in the queue-fed kernel: 0 FP64 (default); 1 FP32
