@@ -146,3 +146,31 @@ def test_autograd_function_gradients(wr, adj):
     og0, omb = adj.backward(tape, sNo.grad.numpy())
     assert np.array_equal(_bits(og0), _bits(gs0))
     assert np.array_equal(_bits(omb.sum(axis=1)), _bits(gth))
+
+
+def test_record_errors_are_reported(wr):
+    """A tape too small for the run's accepted steps is WR_ERR_CAPACITY; a
+    cell that stops integrating (its density driver past the collapse stop,
+    reproducer.cpp:7752) is WR_ERR_ARGUMENT; neither leaves a usable tape."""
+    from importlib import import_module
+    rt = import_module("warp-reactor_amd.runtime")
+    n = 8
+    _, sigma, t0, t1 = wr.RECIPES["uniform16"]
+    cells = wr.make_recipe_states(n, 2, sigma, t0, t1)
+    ones = np.ones(wr.adjoint_ntheta())
+    with wr.Context(n) as ctx:
+        ctx.upload(cells)
+        with pytest.raises(wr.WarpReactorError) as e:
+            ctx.adjoint_record(NSTEPS, ones, hcap=3)
+        assert e.value.code == rt.ERR_CAPACITY
+        with pytest.raises(wr.WarpReactorError):
+            ctx.adjoint_backward(np.zeros((18, n)))
+        stopped = cells.copy()
+        stopped["density_driver"][3] = 3.0e-6
+        ctx.upload(stopped)
+        with pytest.raises(wr.WarpReactorError) as e:
+            ctx.adjoint_record(5, ones)
+        assert e.value.code == rt.ERR_ARGUMENT
+        # a good recording afterwards on the same context
+        ctx.upload(cells)
+        assert ctx.adjoint_record(NSTEPS, ones) == NSTEPS

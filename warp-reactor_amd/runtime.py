@@ -369,7 +369,7 @@ class Context:
         done = ctypes.c_int()
         r = self._lib.wr_adjoint_record(self._h, int(nsteps), _ptr(M), int(hcap), ctypes.byref(done))
         if r != SUCCESS:
-            _check(r if r <= ERR_HIP else ERR_ARGUMENT, f"wr_adjoint_record (status {r})")
+            _check(r, "wr_adjoint_record")
         self._adj = (int(nsteps), int(hcap))
         return done.value
 
@@ -381,7 +381,7 @@ class Context:
         r = self._lib.wr_adjoint_replay(self._h, _ptr(a) if a is not None else None,
                                         _ptr(m) if m is not None else None, _ptr(sN))
         if r != SUCCESS:
-            _check(r if r <= ERR_HIP else ERR_ARGUMENT, f"wr_adjoint_replay (status {r})")
+            _check(r, "wr_adjoint_replay")
         return sN
 
     def adjoint_backward(self, gN):
@@ -391,7 +391,7 @@ class Context:
         g = np.ascontiguousarray(gN, dtype=np.float64)
         r = self._lib.wr_adjoint_backward(self._h, _ptr(g), _ptr(g0), _ptr(mb))
         if r != SUCCESS:
-            _check(r if r <= ERR_HIP else ERR_ARGUMENT, f"wr_adjoint_backward (status {r})")
+            _check(r, "wr_adjoint_backward")
         return g0, mb
 
     def adjoint_tape(self):
