@@ -77,14 +77,6 @@ __device__ __forceinline__ void adj_error(int* err, int code) { atomicCAS(err, 0
 
 // The generated pieces as real (non-inlined) functions: each body is compiled
 // once, whatever the number of call sites.  w: this thread's workspace column.
-__device__ __noinline__ void theta_fj(const double* y, const double* M, double* w, int st,
-                                      double* J) {
-    ad_fj_st(y, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVF) * st, J);
-}
-__device__ __noinline__ void theta_f(const double* y, const double* M, double* w, int st,
-                                     double* out) {
-    ad_f_st(y, 1, M, 1, w + kWSTG * st, st, out, nullptr);
-}
 // ---- the recording run in the queue-fed kernel: f_theta, checkpoint, tape ----------------
 struct QThetaNet {
     static constexpr int kSlots = AD_FWD_NSLOT;
@@ -137,10 +129,10 @@ __global__ void k_adj_dts(const Ctl* ctl, double* dts, int step) {
 // ---- one accepted ROS2S step at a fixed h -------------------------------------------------
 // The attempt of the integrator (reproducer.cpp:7260-7302), same operations in the
 // same order; J, LU, k1..k3, u2, u3 stay in the workspace for the adjoint.
-__device__ __noinline__ int adj_substep(double (&y)[kNeqs], double h, const double* M, double* w,
-                                       int st, uint64_t& piv) {
+__device__ __forceinline__ int adj_substep(double (&y)[kNeqs], double h, const double* M,
+                                           double* w, int st, uint64_t& piv) {
     using C = Ros2sCoef;
-    theta_fj(y, M, w, st, w + kWJ * st);
+    ad_fj_st(y, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVF) * st, w + kWJ * st);
     ThreadWs tw;
     tw.J = w + kWJ * st;
     tw.LU = w + kWLU * st;
@@ -149,35 +141,35 @@ __device__ __noinline__ int adj_substep(double (&y)[kNeqs], double h, const doub
     tw.stride = st;
     const double fac = 1.0 / (h * C::gamma);
     if (lu_left(tw, fac, piv) != 0) return kLuError;
-    double v[kNeqs], u[kNeqs], k1[kNeqs];
+    double v[kNeqs], u[kNeqs], k1[kNeqs], k2[kNeqs];
 #pragma unroll
     for (int i = 0; i < kNeqs; ++i) v[i] = w[(kWV + kVF + i) * st];
     lu_solve_ws(tw, piv, v);
 #pragma unroll
-    for (int i = 0; i < kNeqs; ++i) {
-        k1[i] = v[i];
-        w[(kWV + kVK1 + i) * st] = v[i];
-        u[i] = y[i] + C::a21 * v[i];
-        w[(kWV + kVU2 + i) * st] = u[i];
-        v[i] = (C::c21 / h) * v[i];
+    for (int i = 0; i < kNeqs; ++i) k1[i] = v[i];
+    // stages 2 and 3 through one loop body (one instance of the generated f)
+#pragma unroll 1
+    for (int stage = 2; stage <= 3; ++stage) {
+        const int kf = stage == 2 ? kVK1 : kVK2;
+        const int uf = stage == 2 ? kVU2 : kVU3;
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) {
+            w[(kWV + kf + i) * st] = v[i];
+            if (stage == 2) {
+                u[i] = y[i] + C::a21 * v[i];
+                v[i] = (C::c21 / h) * v[i];
+            } else {
+                k2[i] = v[i];
+                u[i] = y[i] + C::a31 * k1[i] + C::a32 * k2[i];
+                v[i] = (C::c31 * k1[i] + C::c32 * k2[i]) / h;
+            }
+            w[(kWV + uf + i) * st] = u[i];
+        }
+        ad_f_st(u, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVDY) * st, nullptr);
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) v[i] += w[(kWV + kVDY + i) * st];
+        lu_solve_ws(tw, piv, v);
     }
-    theta_f(u, M, w, st, w + (kWV + kVDY) * st);
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) v[i] += w[(kWV + kVDY + i) * st];
-    lu_solve_ws(tw, piv, v);
-    double k2[kNeqs];
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) {
-        k2[i] = v[i];
-        w[(kWV + kVK2 + i) * st] = v[i];
-        u[i] = y[i] + C::a31 * k1[i] + C::a32 * k2[i];
-        w[(kWV + kVU3 + i) * st] = u[i];
-        v[i] = (C::c31 * k1[i] + C::c32 * k2[i]) / h;
-    }
-    theta_f(u, M, w, st, w + (kWV + kVDY) * st);
-#pragma unroll
-    for (int i = 0; i < kNeqs; ++i) v[i] += w[(kWV + kVDY + i) * st];
-    lu_solve_ws(tw, piv, v);
 #pragma unroll
     for (int i = 0; i < kNeqs; ++i) {
         w[(kWV + kVK3 + i) * st] = v[i];
@@ -288,11 +280,12 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
 
 // ---- the frozen map forward ------------------------------------------------------------
 __global__ __launch_bounds__(kAdjBlock) void k_adj_replay(AdjDev a, const double* __restrict__ s0,
-                                                          double* __restrict__ sN, int n) {
+                                                          double* __restrict__ sN, int n, int zero) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    double* w = a.ws + c;
-    const int st = a.cap;
+    double priv[kWsSteps];
+    double* const w = priv + zero;
+    const int st = 1;
     double sv[kNF];
 #pragma unroll
     for (int f = 0; f < kNF; ++f) sv[f] = s0[static_cast<size_t>(f) * a.cap + c];
@@ -327,14 +320,16 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_replay(AdjDev a, const double
 // step from its checkpoint and store each accepted substep's pieces in slot
 // j = k - hoff[s0] of the cell (sub[(j * kSub + field) * cap + cell]).  The
 // thread's workspace column (same layout as the per-cell one) has stride nA.
-__global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __restrict__ wA,
-                                                         int nA, double* __restrict__ sub, int n,
-                                                         int s0, int s1) {
+__global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __restrict__ sub, int n,
+                                                         int s0, int s1, int zero) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * (s1 - s0)) return;
     const int c = t % n;
     const int step = s0 + t / n;
-    double* w = wA + t;
+    // the step's workspace in lane-private memory (`zero` is always 0)
+    double priv[kWsSteps];
+    double* const w = priv + zero;
+    const int nA = 1;
     double sv[kNF];
     const double* ck = a.ckpt + static_cast<size_t>(step) * kNF * a.cap + c;
 #pragma unroll
@@ -447,8 +442,6 @@ struct AdjointState {
     std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
-    double* wA = nullptr;     // workspace of the step-recompute threads
-    size_t wA_elems = 0;
     bool recorded = false;
     double* ckpt = nullptr;
     int* hoff = nullptr;
@@ -480,7 +473,6 @@ void wr::detail::adjoint_destroy(wr_context* ctx) {
     (void)hipFree(a->Mr);
     (void)hipFree(a->ws);
     (void)hipFree(a->sub);
-    (void)hipFree(a->wA);
     (void)hipFree(a->err);
     (void)hipFree(a->io);
     if (a->ev0) (void)hipEventDestroy(a->ev0);
@@ -710,7 +702,7 @@ int wr_adjoint_replay(wr_context* ctx, const double* s0, const double* multiplie
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
     hipLaunchKernelGGL(k_adj_replay, dim3(blocks_for(a->n, kAdjBlock)), dim3(kAdjBlock), 0,
-                       ctx->stream, dev_view(a, M), d_s0, d_sN, a->n);
+                       ctx->stream, dev_view(a, M), d_s0, d_sN, a->n, 0);
     WR_HIP(hipGetLastError(), "k_adj_replay launch");
     if ((st = timed_end(ctx, a)) != WR_SUCCESS) return st;
     if ((st = get_fields(ctx, sN, d_sN, kNF)) != WR_SUCCESS) return st;
@@ -748,8 +740,7 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
                               a->hoff_h[static_cast<size_t>(lo) * cap + c];
                 if (k > most) most = k;
             }
-            return (static_cast<size_t>(most) * kSub * cap +
-                    static_cast<size_t>(s1 - lo) * n * kWsSteps) * sizeof(double);
+            return static_cast<size_t>(most) * kSub * cap * sizeof(double);
         };
         while (s0 > 0 && bytes(s0 - 1) <= kWindowBudgetBytes) --s0;
         int most = 1;
@@ -760,12 +751,10 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
         }
         const int nA = n * (s1 - s0);
         if ((st = ensure_buf(&a->sub, &a->sub_elems, static_cast<size_t>(most) * kSub * cap,
-                             "hipMalloc(adjoint substeps)")) != WR_SUCCESS ||
-            (st = ensure_buf(&a->wA, &a->wA_elems, static_cast<size_t>(nA) * kWsSteps,
-                             "hipMalloc(adjoint step workspace)")) != WR_SUCCESS)
+                             "hipMalloc(adjoint substeps)")) != WR_SUCCESS)
             return st;
         hipLaunchKernelGGL(k_adj_steps, dim3(blocks_for(nA, kAdjBlock)), dim3(kAdjBlock), 0,
-                           ctx->stream, d, a->wA, nA, a->sub, n, s0, s1);
+                           ctx->stream, d, a->sub, n, s0, s1, 0);
         WR_HIP(hipGetLastError(), "k_adj_steps launch");
         hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, kAdjBlock)), dim3(kAdjBlock), 0,
                            ctx->stream, d, a->sub, d_lam, n, s0, s1, s1 == a->nsteps ? 1 : 0, 0);
