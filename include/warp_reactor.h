@@ -244,6 +244,11 @@ int wr_adjoint_backward(wr_context *ctx, const double *gN, double *g0, double *m
  * htape [hcap][n], dts [nsteps], ckpt [nsteps][18][n]; any may be NULL. */
 int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double *dts,
                     double *ckpt);
+/* The backward pass takes the recorded steps in windows from the last one
+ * backwards, each window's accepted steps stored at once (sized to a 24 GiB
+ * budget); max_steps > 0 caps the steps per window (0: the budget alone).
+ * Results do not depend on it. */
+int wr_adjoint_set_window(wr_context *ctx, int max_steps);
 /* Device time (HIP events on the context's stream) of the last record,
  * replay or backward call, ms. */
 int wr_adjoint_last_ms(const wr_context *ctx, double *ms);

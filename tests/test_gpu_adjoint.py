@@ -174,3 +174,19 @@ def test_record_errors_are_reported(wr):
         # a good recording afterwards on the same context
         ctx.upload(cells)
         assert ctx.adjoint_record(NSTEPS, ones) == NSTEPS
+
+
+def test_backward_windows_give_the_same_gradients(recorded):
+    """The backward pass over windows of 3, 7 and 1 steps (the stored substeps
+    of one window at a time) equals the single-window pass bit for bit."""
+    ctx, _, _, _, tape, _ = recorded
+    rng = np.random.default_rng(25)
+    gN = rng.standard_normal((18, tape["n"]))
+    ctx.adjoint_set_window(0)
+    g0, mb = ctx.adjoint_backward(gN)
+    for w in (3, 7, 1):
+        ctx.adjoint_set_window(w)
+        g0w, mbw = ctx.adjoint_backward(gN)
+        assert np.array_equal(_bits(g0w), _bits(g0)), w
+        assert np.array_equal(_bits(mbw), _bits(mb)), w
+    ctx.adjoint_set_window(0)

@@ -439,6 +439,7 @@ __global__ void k_adj_mbar(AdjDev a, double* __restrict__ mbar, int n) {
 // ---- host state ------------------------------------------------------------------------------
 struct AdjointState {
     int nsteps = 0, hcap = 0, cap = 0, n = 0;
+    int window_max = 0;       // most steps per backward window (0: by memory budget)
     std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
@@ -742,7 +743,9 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
             }
             return static_cast<size_t>(most) * kSub * cap * sizeof(double);
         };
-        while (s0 > 0 && bytes(s0 - 1) <= kWindowBudgetBytes) --s0;
+        while (s0 > 0 && bytes(s0 - 1) <= kWindowBudgetBytes &&
+               (a->window_max <= 0 || s1 - (s0 - 1) <= a->window_max))
+            --s0;
         int most = 1;
         for (int c = 0; c < n; ++c) {
             const int k = a->hoff_h[static_cast<size_t>(s1) * cap + c] -
@@ -770,6 +773,13 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     int code = 0;
     if ((st = read_err(ctx, a, &code)) != WR_SUCCESS) return st;
     return code == 0 ? kSuccess : kLuError;
+}
+
+int wr_adjoint_set_window(wr_context* ctx, int max_steps) {
+    if (!ctx || max_steps < 0) return fail(WR_ERR_ARGUMENT, "wr_adjoint_set_window: bad argument");
+    if (!ctx->adj) ctx->adj = new AdjointState();
+    ctx->adj->window_max = max_steps;
+    return WR_SUCCESS;
 }
 
 int wr_adjoint_last_ms(const wr_context* ctx, double* ms) {

@@ -102,6 +102,7 @@ _SIGS = {
     "wr_adjoint_backward": (_I, [_VP, _VP, _VP, _VP]),
     "wr_adjoint_tape": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "wr_adjoint_last_ms": (_I, [_VP, ctypes.POINTER(_D)]),
+    "wr_adjoint_set_window": (_I, [_VP, _I]),
     "wr_make_collapse_states": (_I, [_VP, _I]),
     "wr_make_recipe_states": (_I, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
     "wr_pack_final_states": (_I, [_VP, _I, _I, _VP]),
@@ -402,6 +403,10 @@ class Context:
         _check(self._lib.wr_adjoint_tape(self._h, _ptr(t["hoff"]), _ptr(t["htape"]), _ptr(t["dts"]),
                                          _ptr(t["ckpt"])), "wr_adjoint_tape")
         return t
+
+    def adjoint_set_window(self, max_steps=0):
+        """Cap the steps per backward window (0: by the memory budget)."""
+        _check(self._lib.wr_adjoint_set_window(self._h, int(max_steps)), "wr_adjoint_set_window")
 
     def adjoint_last_ms(self):
         ms = ctypes.c_double()
