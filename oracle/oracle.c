@@ -885,6 +885,55 @@ int or_run_gridwide(or_cell *cells, int n, int perturb, int max_steps, int *comp
     return result;
 }
 
+/* The reference's CPU driver, legacy-local policy (reproducer.cpp:8630-8648):
+ * every outer step each cell that has not stopped takes one step of its own
+ * free-fall dt (advance_collapse_step, 7646-7685); the cell loop stops at the
+ * first failing cell (cells after it keep their state), the run stops on a
+ * failure or when every cell stopped.  max_steps <= 0 means 1000. */
+static int or_advance_local(or_cell *s, int cell, int step, int perturb, int *failure) {
+    or_apply_perturbation(s, cell, step, perturb);
+    const double old_density = s->density_driver;
+    const double dt = or_collapse_timestep(s);
+    const double tff = dt / 1.0e-1;
+    s->density_driver += dt * (s->density_driver / tff);
+    if (dt < 10.0 || s->density_driver > 2.0e-6) return 1;
+    const double ratio = s->density_driver / old_density;
+    for (int n = 0; n < NSPEC; ++n) s->current.xn[n] *= ratio;
+    s->current.rho *= ratio;
+    const int r = or_burn_ros2s(&s->current, dt, &s->stats);
+    if (r != OR_SUCCESS) {
+        *failure = r;
+        return 1;
+    }
+    or_post_step((double *)&s->current);
+    s->time += dt;
+    s->completed_steps += 1;
+    return 0;
+}
+
+int or_run_legacy(or_cell *cells, int n, int perturb, int max_steps, int *completed_out,
+                  int *failed_cell) {
+    const int limit = max_steps > 0 ? max_steps : 1000;
+    int failure = OR_SUCCESS, completed = 0;
+    *failed_cell = -1;
+    for (int step = 0; step < limit; ++step) {
+        int all_stopped = 1;
+        for (int c = 0; c < n; ++c) {
+            if (cells[c].completed_steps < completed) continue;
+            const int stopped = or_advance_local(&cells[c], c, step, perturb, &failure);
+            if (failure != OR_SUCCESS) {
+                *failed_cell = c;
+                break;
+            }
+            all_stopped = all_stopped && stopped;
+        }
+        if (failure != OR_SUCCESS || all_stopped) break;
+        completed += 1;
+    }
+    *completed_out = completed;
+    return failure;
+}
+
 int or_sizeof_cell(void) { return (int)sizeof(or_cell); }
 
 /* eos_rt on a burn_t laid out as rho, T, e, xn[14] (reproducer.cpp:306-309) */

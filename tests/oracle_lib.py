@@ -45,6 +45,7 @@ class _Cpu:
         if p == "or_":
             sig.update({"set_integrator_max_steps": (None, [_I]),
                         "set_lu_precision": (None, [_I, _I]),
+                        "run_legacy": (_I, [_VP, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
                         "mixed_fallbacks": (None, [_VP, _VP]),
                         "reset_mixed_fallbacks": (None, []),
                         "lu32": (_I, [_VP, _VP]), "lu_solve32": (None, [_VP, _VP, _VP]),
@@ -153,6 +154,14 @@ class _Cpu:
             self._make_recipe_state(_VP(base + i * item), i, seed, sigma, t_min, t_max)
         return cells
 
+    def run_legacy(self, cells, perturb=True, max_steps=0):
+        """The legacy-local driver (reference CPU policy); (status, completed, failed cell)."""
+        done, failed = _I(0), _I(0)
+        r = self._run_legacy(_p(cells), len(cells), int(perturb), max_steps, ctypes.byref(done),
+                             ctypes.byref(failed))
+        cells["reserved_"] = 0
+        return r, done.value, failed.value
+
     def set_lu_precision(self, mode, sweeps=2):
         """0: FP64 factors; 1: FP32 factors + FP64 refinement (oracle global)."""
         self._set_lu_precision(int(mode), int(sweeps))
@@ -226,6 +235,16 @@ class Reference(_Cpu):
         if not os.path.exists(REF_SO):
             return None
         return cls(REF_SO)
+
+    def run_legacy(self, cells, perturb=True):
+        """The reference's own CPU driver loop (legacy-local policy)."""
+        fn = self.lib.ref_run_legacy
+        fn.restype = _I
+        fn.argtypes = [_VP, _I, _I, ctypes.POINTER(_I)]
+        done = _I(0)
+        r = fn(_p(cells), len(cells), int(perturb), ctypes.byref(done))
+        cells["reserved_"] = 0
+        return r, done.value
 
 
 class OracleAdjoint(OracleWrMath):
