@@ -162,6 +162,20 @@ int wr_prepare(wr_context *ctx, int completed_global_steps, double grid_time, in
 int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_time,
                double dt_grid, int *integrated);
 
+/* Driver policy of subsequent wr_run calls.  WR_POLICY_GRIDWIDE (default):
+ * the gridwide-independent loop of run_cells_cuda (reproducer.cpp:7791-7975),
+ * one grid-wide dt per outer step.  WR_POLICY_LEGACY_LOCAL: the reference's
+ * CPU driver (reproducer.cpp:8628-8648): every outer step each cell that has
+ * not stopped applies its perturbation and takes one step of its own
+ * free-fall dt (advance_collapse_step, 7646-7685); the run stops when every
+ * cell stopped or at the first failure, and, as the reference's sequential
+ * cell loop stops at the lowest failing cell, the cells after it keep their
+ * state of that step's start.  grid_time is not used (each cell keeps its own
+ * time); wr_dt_history records no steps.  The legacy-local policy runs on the
+ * queue-fed kernel in FP64 (FP32 factorisation: WR_ERR_ARGUMENT). */
+enum wr_policy { WR_POLICY_GRIDWIDE = 0, WR_POLICY_LEGACY_LOCAL = 1 };
+int wr_set_policy(wr_context *ctx, int policy);
+
 /* Advance kernel of subsequent steps: 1 (default) = queue-fed persistent
  * kernel (lanes take the next cell when theirs finishes; register-budgeted
  * network code); 0 = one thread per cell.  Results are identical. */

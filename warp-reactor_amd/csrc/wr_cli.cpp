@@ -1,12 +1,16 @@
 // wr_reproducer: command-line front end with the reference reproducer's
-// interface and output (reproducer.cpp:8201-8277, 8591-8703) for the
-// gridwide-independent policy on the gfx950 library.
+// interface and output (reproducer.cpp:8201-8277, 8591-8703) on the gfx950
+// library.  The driver policy is the reference's HIP build's
+// (gridwide-independent) unless --driver-policy legacy-local asks for its CPU
+// build's (each cell steps with its own free-fall dt, 8628-8648).
 //
 //   wr_reproducer [--grid N] [--perturb|--no-perturb]
 //                 [--compare-final-state FILE|--no-compare-final-state]
+//                 [--driver-policy gridwide-independent|legacy-local]
 //
-// Writes final_states_grid<N>_hip.bin (172-byte records) for N > 1, prints
-// the final state for N = 1, exit status 0 on success and 1 on failure.
+// Writes final_states_grid<N>_hip.bin (_hip_legacy_local.bin for the
+// legacy-local policy; 172-byte records) for N > 1, prints the final state
+// for N = 1, exit status 0 on success and 1 on failure.
 #include <errno.h>
 #include <limits.h>
 #include <stdio.h>
@@ -30,13 +34,15 @@ struct Options {
     int grid_dim = 64;
     bool perturb = true;
     bool show_help = false;
+    bool legacy_local = false;
     std::string compare_path;
 };
 
 void print_usage(const char* program) {
     fprintf(stderr,
             "usage: %s [--grid N] [--perturb|--no-perturb]"
-            " [--compare-final-state FILE|--no-compare-final-state]\n",
+            " [--compare-final-state FILE|--no-compare-final-state]"
+            " [--driver-policy gridwide-independent|legacy-local]\n",
             program);
 }
 
@@ -78,6 +84,15 @@ bool parse_args(int argc, char** argv, Options& o) {
                 return false;
             }
             o.compare_path = argv[++i];
+            continue;
+        }
+        if (arg == "--driver-policy") {
+            const std::string v = i + 1 < argc ? argv[++i] : "";
+            if (v != "gridwide-independent" && v != "legacy-local") {
+                print_usage(argv[0]);
+                return false;
+            }
+            o.legacy_local = v == "legacy-local";
             continue;
         }
         if (arg == "--no-compare-final-state") {
@@ -134,6 +149,28 @@ CounterSummary summarize_counter(const std::vector<wr_collapse_state>& cells, si
     return s;
 }
 
+// The reference's CPU driver loop (reproducer.cpp:8628-8648) on the device:
+// context API with the legacy-local policy; HIP and API errors are reported
+// on stderr and fail the run.
+int run_legacy_local(std::vector<wr_collapse_state>& cells, bool perturb, int& completed) {
+    const int n = static_cast<int>(cells.size());
+    wr_context* ctx = nullptr;
+    int r = wr_context_create(0, n, &ctx);
+    if (r == WR_SUCCESS) r = wr_set_policy(ctx, WR_POLICY_LEGACY_LOCAL);
+    if (r == WR_SUCCESS) r = wr_upload_cells(ctx, cells.data(), n);
+    if (r == WR_SUCCESS) r = wr_run(ctx, perturb ? 1 : 0, 1000, 0, 0.0);
+    wr_run_info info{};
+    if (r > WR_ERR_HIP && ctx) {
+        int d = wr_download_cells(ctx, cells.data(), n);
+        if (d == WR_SUCCESS) d = wr_run_info_get(ctx, &info);
+        if (d != WR_SUCCESS) r = d;
+        completed = info.completed_global_steps;
+    }
+    if (r <= WR_ERR_HIP) fprintf(stderr, "%s\n", wr_last_error());
+    wr_context_destroy(ctx);
+    return r;
+}
+
 int finish(bool pass, bool emit_status) {
     fflush(stderr);
     if (emit_status) printf("\n%s\n", pass ? "PASS" : "FAIL");
@@ -161,7 +198,9 @@ int main(int argc, char** argv) {
 
     int completed = 0;
     const auto t0 = std::chrono::steady_clock::now();
-    const int result = wr_run_cells(cells.data(), num_cells, o.perturb ? 1 : 0, &completed);
+    const int result = o.legacy_local ? run_legacy_local(cells, o.perturb, completed)
+                                      : wr_run_cells(cells.data(), num_cells, o.perturb ? 1 : 0,
+                                                     &completed);
     const auto t1 = std::chrono::steady_clock::now();
     const double elapsed = std::chrono::duration<double>(t1 - t0).count();
     if (result != WR_SUCCESS) return finish(false, false);
@@ -181,7 +220,7 @@ int main(int argc, char** argv) {
     printf("Primordial chemistry collapse grid with ROS2S\n");
     printf("grid: %d^3 (%d cells)\n", o.grid_dim, num_cells);
     printf("perturbations: %s\n", o.perturb ? "enabled" : "disabled");
-    printf("driver policy: gridwide-independent\n");
+    printf("driver policy: %s\n", o.legacy_local ? "legacy-local" : "gridwide-independent");
     printf("completed global collapse steps: %d\n", completed);
     if (num_cells == 1) {
         printf("representative cell completed steps: %d\n", rep.completed_steps);
@@ -244,7 +283,8 @@ int main(int argc, char** argv) {
     }
 
     if (o.grid_dim > 1) {
-        const std::string file = "final_states_grid" + std::to_string(o.grid_dim) + "_" + kBackend + ".bin";
+        const std::string file = "final_states_grid" + std::to_string(o.grid_dim) + "_" + kBackend +
+                                 (o.legacy_local ? "_legacy_local" : "") + ".bin";
         if (wr_write_final_states(cells.data(), num_cells, o.grid_dim, file.c_str()) != WR_SUCCESS)
             return finish(false, comparison_done);
         printf("final states: %s (%d packed records, %zu bytes each)\n", file.c_str(), num_cells,

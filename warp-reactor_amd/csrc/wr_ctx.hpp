@@ -60,6 +60,9 @@ struct Ctl {
     int solo_lanes;       // queue positions reserved for block 0's first wave (0: none)
     unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
     unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
+    // legacy-local policy: (failing cell << 32 | code) of the step's lowest
+    // failing cell, all ones when none failed
+    unsigned long long local_fail_key;
     int bin_count[16];  // cells per attempt-count bin, for the queue order
     int bin_used[16];   // queue positions already handed to blocks, per bin
 };
@@ -142,6 +145,11 @@ struct wr_context {
     hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
     double adv_ms = 0.0;
     int adv_launches = 0;
+    // driver policy of wr_run: WR_POLICY_GRIDWIDE (default) or WR_POLICY_LEGACY_LOCAL
+    int policy = 0;
+    size_t cell_bytes = 0;           // bytes of cells_mem
+    void* local_backup = nullptr;    // cells at the start of a legacy-local step
+    int q_blocks_local = 0;          // persistent grid of the legacy-local kernel
     wr_run_info info{};
     wr::AdjointState* adj = nullptr;  // created on first use by the adjoint entry points
 };

@@ -375,6 +375,68 @@ __global__ void k_commit(Ctl* ctl) {
     }
 }
 
+// ---- legacy-local policy (reproducer.cpp:8628-8648) ----------------------------------
+// One outer step: k_local_begin, a copy of the cells, the queue order,
+// k_advance_q<.., kLocal>, k_local_restore, k_local_commit.
+__global__ void k_local_begin(Ctl* ctl) {
+    if (ctl->done) return;
+    ctl->integrated = 0;
+    ctl->queue = 0;
+    ctl->queue_solo = 0;
+    ctl->local_fail_key = ~0ull;
+}
+
+// The reference's cell loop stops at the first failing cell (8636-8638): the
+// cells after it keep their state of the step's start, taken from the copy b.
+__global__ __launch_bounds__(kOrderBlock) void k_local_restore(DevCells s, DevCells b,
+                                                                const Ctl* ctl) {
+    if (ctl->done) return;
+    const unsigned long long key = ctl->local_fail_key;
+    if (key == ~0ull) return;
+    const int f = static_cast<int>(key >> 32);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= f || c >= s.n) return;
+    for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = b.xn[n * b.cap + c];
+    s.rho[c] = b.rho[c];
+    s.T[c] = b.T[c];
+    s.e[c] = b.e[c];
+    s.time[c] = b.time[c];
+    s.driver[c] = b.driver[c];
+    s.completed[c] = b.completed[c];
+    for (int k = 0; k < 7; ++k) s.stats[k * s.cap + c] = b.stats[k * b.cap + c];
+}
+
+// End of the outer step (8639-8642): failure, every cell stopped, or next step.
+__global__ void k_local_commit(Ctl* ctl) {
+    if (ctl->done) return;
+    for (int b = 0; b < kCostBins; ++b) {
+        ctl->bin_count[b] = 0;
+        ctl->bin_used[b] = 0;
+    }
+    const unsigned long long key = ctl->local_fail_key;
+    if (key != ~0ull) {
+        ctl->failure = static_cast<int>(key & 0xffffffffull);
+        ctl->failed_cell = static_cast<int>(key >> 32);
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_FAILURE;
+        ctl->failed_step = ctl->step;
+        ctl->failed_phase = 2;
+        return;
+    }
+    if (ctl->integrated == 0) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_NONE_INTEGRATED;
+        return;
+    }
+    ctl->cell_advances += static_cast<unsigned long long>(ctl->integrated);
+    ctl->completed += 1;
+    ctl->step += 1;
+    if (ctl->step >= ctl->step_limit) {
+        ctl->done = 1;
+        ctl->stop_reason = WR_STOP_STEP_LIMIT;
+    }
+}
+
 // ---- building-block kernels (verification entry points) ------------------------------
 __global__ void k_eval_rhs(const double* __restrict__ y, int n, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,6 +599,7 @@ void wr_context_destroy(wr_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->cells_mem);
+    (void)hipFree(ctx->local_backup);
     (void)hipFree(ctx->ws_mem);
     (void)hipFree(ctx->ctl);
     (void)hipFree(ctx->part_dt);
@@ -571,6 +634,7 @@ int wr_context_create(int device, int capacity_cells, wr_context** out) {
         return cleanup(WR_ERR_HIP);
     const size_t cell_bytes = cap * (sizeof(double) * (kNumSpec + 5) + sizeof(int) * 3 +
                                      sizeof(unsigned long long) * 7) + 4096;
+    ctx->cell_bytes = cell_bytes;
     if (!hip_ok(hipMalloc(&ctx->cells_mem, cell_bytes), "hipMalloc(cells)"))
         return cleanup(WR_ERR_HIP);
     char* p = static_cast<char*>(ctx->cells_mem);
@@ -807,11 +871,106 @@ static int chunk_graph(wr_context* ctx) {
     return WR_SUCCESS;
 }
 
+// The cells' arrays at the same offsets in another allocation of cells_mem's size.
+static DevCells cells_at(const wr_context* ctx, void* base) {
+    const auto move = [&](auto* q) {
+        const char* p = reinterpret_cast<const char*>(q);
+        return reinterpret_cast<decltype(q)>(static_cast<char*>(base) +
+                                             (p - static_cast<const char*>(ctx->cells_mem)));
+    };
+    DevCells b = ctx->cells;
+    b.xn = move(b.xn);
+    b.rho = move(b.rho);
+    b.T = move(b.T);
+    b.e = move(b.e);
+    b.time = move(b.time);
+    b.driver = move(b.driver);
+    b.completed = move(b.completed);
+    b.stats = move(b.stats);
+    b.cost = move(b.cost);
+    b.order = move(b.order);
+    return b;
+}
+
+// Outer loop of the legacy-local policy, 8 steps per host poll (no graph: the
+// step copies the cells with hipMemcpyAsync).
+static int run_legacy(wr_context* ctx, int max_steps) {
+    if (!ctx->local_backup &&
+        !hip_ok(hipMalloc(&ctx->local_backup, ctx->cell_bytes), "hipMalloc(legacy-local copy)"))
+        return WR_ERR_HIP;
+    if (ctx->q_blocks_local == 0) {
+        int cus = 0, per_cu = 0;
+        WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+               "hipDeviceGetAttribute(CU count)");
+        WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &per_cu, k_advance_q<false, QPlainNet, true>, ctx->q_threads, 0),
+               "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        if (per_cu < 1) per_cu = 1;
+        if (ctx->q_per_cu > 0) per_cu = ctx->q_per_cu;
+        int blocks = cus * per_cu;
+        const int need = blocks_for(ctx->cap, ctx->q_threads);
+        if (blocks > need) blocks = need;
+        ctx->q_blocks_local = blocks < 1 ? 1 : blocks;
+    }
+    const DevCells backup = cells_at(ctx, ctx->local_backup);
+    const int nb = blocks_for(ctx->n, kOrderBlock);
+    const int chunk = wr_context::kChunk;
+    for (int issued = 0; issued < max_steps;) {
+        const int todo = (max_steps - issued < chunk) ? max_steps - issued : chunk;
+        const int steps_before = ctx->ctl_host->step;
+        for (int k = 0; k < todo; ++k) {
+            hipLaunchKernelGGL(k_local_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
+            WR_HIP(hipMemcpyAsync(ctx->local_backup, ctx->cells_mem, ctx->cell_bytes,
+                                  hipMemcpyDeviceToDevice, ctx->stream),
+                   "hipMemcpyAsync(legacy-local copy)");
+            hipLaunchKernelGGL(k_order_count, dim3(nb), dim3(kOrderBlock), 0, ctx->stream,
+                               ctx->cells, ctx->ctl);
+            hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(kOrderBlock), 0, ctx->stream,
+                               ctx->cells, ctx->ctl);
+            if (ctx->timing)
+                WR_HIP(hipEventRecord(ctx->adv_start[k], ctx->stream), "hipEventRecord");
+            hipLaunchKernelGGL((k_advance_q<false, QPlainNet, true>), dim3(ctx->q_blocks_local),
+                               dim3(ctx->q_threads), 0, ctx->stream, ctx->cells, ctx->ctl, 0,
+                               QPlainNet{});
+            if (ctx->timing)
+                WR_HIP(hipEventRecord(ctx->adv_stop[k], ctx->stream), "hipEventRecord");
+            hipLaunchKernelGGL(k_local_restore, dim3(nb), dim3(kOrderBlock), 0, ctx->stream,
+                               ctx->cells, backup, ctx->ctl);
+            hipLaunchKernelGGL(k_local_commit, dim3(1), dim3(1), 0, ctx->stream, ctx->ctl);
+            WR_HIP(hipGetLastError(), "legacy-local step launch");
+        }
+        issued += todo;
+        int st;
+        if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
+        if (ctx->timing) {
+            const Ctl& h = *ctx->ctl_host;
+            int ran = h.step - steps_before;
+            if (h.done && (h.stop_reason == WR_STOP_NONE_INTEGRATED ||
+                           (h.stop_reason == WR_STOP_FAILURE && h.failed_phase == 2)))
+                ran += 1;
+            for (int k = 0; k < ran && k < todo; ++k) {
+                float ms = 0.0f;
+                WR_HIP(hipEventElapsedTime(&ms, ctx->adv_start[k], ctx->adv_stop[k]),
+                       "hipEventElapsedTime");
+                ctx->adv_ms += ms;
+                ctx->adv_launches += 1;
+            }
+        }
+        if (ctx->ctl_host->done) break;
+    }
+    fill_info(ctx);
+    ctx->hist_len = 0;
+    return ctx->ctl_host->failure;
+}
+
 int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_steps,
            double grid_time) {
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_run: null context");
     if (max_steps <= 0) max_steps = 1000;
     WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    if (ctx->policy == WR_POLICY_LEGACY_LOCAL && ctx->lu_mode != 0)
+        return fail(WR_ERR_ARGUMENT,
+                    "wr_run: the legacy-local policy factorises in FP64 (wr_set_lu_precision 0)");
     int st = init_ctl(ctx, perturb, ctx->n > 0 ? max_steps : 0, completed_global_steps, grid_time);
     if (st != WR_SUCCESS) return st;
     ctx->adv_ms = 0.0;
@@ -821,6 +980,7 @@ int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_ste
         fill_info(ctx);
         return WR_SUCCESS;
     }
+    if (ctx->policy == WR_POLICY_LEGACY_LOCAL) return run_legacy(ctx, max_steps);
     // Steps are enqueued in chunks; kernels of steps past the end return at once.
     const int chunk = wr_context::kChunk;
     for (int issued = 0; issued < max_steps;) {
@@ -870,6 +1030,13 @@ int wr_dt_history(const wr_context* ctx, double* out, int max_entries, int* coun
     return WR_SUCCESS;
 }
 
+int wr_set_policy(wr_context* ctx, int policy) {
+    if (!ctx || (policy != WR_POLICY_GRIDWIDE && policy != WR_POLICY_LEGACY_LOCAL))
+        return fail(WR_ERR_ARGUMENT, "wr_set_policy: policy must be 0 (gridwide) or 1 (legacy-local)");
+    ctx->policy = policy;
+    return WR_SUCCESS;
+}
+
 int wr_set_advance_kernel(wr_context* ctx, int kernel) {
     if (!ctx || kernel < 0 || kernel > 1)
         return fail(WR_ERR_ARGUMENT, "wr_set_advance_kernel: kernel must be 0 or 1");
@@ -899,6 +1066,7 @@ int wr_set_queue_launch(wr_context* ctx, int threads_per_block, int blocks_per_c
     ctx->q_per_cu = blocks_per_cu;
     ctx->q_solo = solo_lanes;
     ctx->q_blocks = 0;
+    ctx->q_blocks_local = 0;
     return WR_SUCCESS;
 }
 

@@ -67,7 +67,11 @@ __device__ __forceinline__ void stage_solve(const ThreadWs& w, const float* lu32
     lu_solve_ws(w, piv, v);
 }
 
-template <bool kMixed, class Net>
+// kLocal: the legacy-local policy (the reference CPU driver, reproducer.cpp:
+// 7646-7685): each cell perturbs itself and steps with its own free-fall dt;
+// a failure is recorded as (cell, code) by atomic minimum, so that the lowest
+// failing cell wins as in the reference's sequential cell loop.
+template <bool kMixed, class Net, bool kLocal = false>
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero, Net net) {
     using C = Ros2sCoef;
     if (ctl->done) return;
@@ -89,6 +93,8 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     const int completed = ctl->completed;
     const double dt_grid = ctl->dt_grid;
     const double next_time = ctl->next_grid_time;
+    const int step = ctl->step;
+    double tout = dt_grid;  // the burn's end time: dt_grid, or (kLocal) the cell's own dt
     const int max_steps = ctl->integ_max_steps;
     int c = -1;
     double x = 0.0, h = 0.0, hopt = 0.0, hacc = 0.0, erracc = 1.0, rho = 0.0, e0 = 0.0;
@@ -111,6 +117,53 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
             }
             if (q >= s.n || ctl->failure != kSuccess) break;
             c = s.order[q];
+            if constexpr (kLocal) {
+                // advance_collapse_step (reproducer.cpp:7646-7665)
+                if (s.completed[c] < completed) {  // stopped in an earlier step
+                    c = -1;
+                    continue;
+                }
+                double xn[kNumSpec];
+#pragma unroll
+                for (int n = 0; n < kNumSpec; ++n) xn[n] = s.xn[n * s.cap + c];
+                double driver = s.driver[c];
+                double rho0 = s.rho[c];
+                double T0 = s.T[c];
+                if (ctl->perturb && step != 0 && step % 20 == 0) {  // apply_perturbation (7564-7580)
+                    const double f = perturbation_factor(c, step);
+                    driver *= f;
+                    rho0 *= f;
+#pragma unroll
+                    for (int n = 0; n < kNumSpec; ++n) xn[n] *= f;
+                    T0 = post_step(xn, rho0, s.e[c]);
+                    s.T[c] = T0;
+                }
+                const double old_density = driver;
+                const double ldt = collapse_timestep(xn);
+                const double tff = ldt / 1.0e-1;
+                driver += ldt * (driver / tff);
+                s.driver[c] = driver;
+                if (ldt < 10.0 || driver > 2.0e-6) {
+#pragma unroll
+                    for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
+                    s.rho[c] = rho0;
+                    c = -1;
+                    continue;
+                }
+                const double ratio = driver / old_density;
+#pragma unroll
+                for (int n = 0; n < kNumSpec; ++n) {
+                    xn[n] *= ratio;
+                    s.xn[n * s.cap + c] = xn[n];
+                }
+                rho = rho0 * ratio;
+                s.rho[c] = rho;
+                e0 = eos_energy(xn, T0);
+#pragma unroll
+                for (int n = 0; n < kNumSpec; ++n) WR_AT(w.vec, kVecY + n, st) = xn[n];
+                WR_AT(w.vec, kVecY + kNumSpec, st) = e0;
+                tout = ldt;
+            } else {
             if (s.completed[c] != completed) {
                 net.skipped(c);
                 c = -1;
@@ -148,10 +201,11 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
 #pragma unroll
             for (int n = 0; n < kNumSpec; ++n) WR_AT(w.vec, kVecY + n, st) = xn[n];
             WR_AT(w.vec, kVecY + kNumSpec, st) = e0;
-            // integrate() prologue, t = 0, tout = dt = dt_grid (> 0, checked by k_plan)
-            h = dt_grid;
+            }
+            // integrate() prologue, t = 0, tout = dt (> 0: checked by k_plan, or >= 10)
+            h = tout;
             if (fabs(h) <= 10.0 * kUround) h = 1.0e-6;
-            h = rmin(fabs(h), dt_grid - 0.0);
+            h = rmin(fabs(h), tout - 0.0);
             x = 0.0;
             hopt = h;
             hacc = 0.0;
@@ -175,8 +229,8 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                 result = kSuccess;
             } else {
                 hopt = h;
-                if (x + h * 1.0001 >= dt_grid) {
-                    h = dt_grid - x;
+                if (x + h * 1.0001 >= tout) {
+                    h = tout - x;
                     last = true;
                 }
                 net.jac_rhs(w);
@@ -279,7 +333,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                         WR_AT(w.vec, kVecY + i, st) = WR_AT(w.vec, kVecYnew + i, st);
                     net.accepted(c, h);
                     x += h;
-                    if (fabs(hnew) > dt_grid) hnew = dt_grid;
+                    if (fabs(hnew) > tout) hnew = tout;
                     if (reject) hnew = rmin(fabs(hnew), fabs(h));
                     reject = false;
                     h = hnew;
@@ -311,12 +365,17 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                 for (int n = 0; n < kNumSpec; ++n) s.xn[n * s.cap + c] = xn[n];
                 s.e[c] = e;
                 s.T[c] = T;
-                s.time[c] = next_time;
+                s.time[c] = kLocal ? s.time[c] + tout : next_time;
                 s.completed[c] = completed + 1;
                 atomicAdd(&ctl->integrated, 1);
             } else {
                 s.e[c] = e0;  // eos_rt ran before the failed burn
-                set_failure(ctl, result, c);
+                if constexpr (kLocal)
+                    atomicMin(&ctl->local_fail_key,
+                              (static_cast<unsigned long long>(c) << 32) |
+                                  static_cast<unsigned int>(result));
+                else
+                    set_failure(ctl, result, c);
             }
             c = -1;
         }

@@ -79,6 +79,7 @@ _SIGS = {
     "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
     "wr_dt_history": (_I, [_VP, _VP, _I, ctypes.POINTER(_I)]),
     "wr_set_advance_kernel": (_I, [_VP, _I]),
+    "wr_set_policy": (_I, [_VP, _I]),
     "wr_set_timing": (_I, [_VP, _I]),
     "wr_set_integrator_max_steps": (_I, [_VP, _I]),
     "wr_set_graph": (_I, [_VP, _I]),
@@ -258,6 +259,13 @@ class Context:
         _check(self._lib.wr_dt_history(self._h, _ptr(out), len(out), ctypes.byref(n)),
                "wr_dt_history")
         return out[:n.value].copy()
+
+    def set_policy(self, policy):
+        """Driver policy of subsequent runs: "gridwide" (default, run_cells_cuda)
+        or "legacy-local" (the reference's CPU driver: each cell steps with its
+        own free-fall dt)."""
+        code = {"gridwide": 0, "legacy-local": 1}.get(policy, policy)
+        _check(self._lib.wr_set_policy(self._h, int(code)), "wr_set_policy")
 
     def set_advance_kernel(self, kernel):
         """1: queue-fed persistent kernel (default); 0: one thread per cell."""
