@@ -25,8 +25,10 @@ import argparse
 import importlib
 import json
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -124,6 +126,12 @@ def main():
                     help="skip the thread-per-cell comparison run")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the FP32-factorisation run")
+    ap.add_argument("--no-jac-reuse", action="store_true",
+                    help="skip the Jacobian-reuse study runs")
+    ap.add_argument("--jac-ages", default="2,4",
+                    help="accepted steps per Jacobian of the Jacobian-reuse study runs")
+    ap.add_argument("--no-legacy", action="store_true",
+                    help="skip the legacy-local policy run")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (32^3, 20 outer steps)")
     ap.add_argument("--no-graph", action="store_true",
@@ -258,6 +266,12 @@ def main():
         ctx.set_advance_kernel(args.advance_kernel)
     if not args.no_mixed and args.advance_kernel == 1:
         line["mixed_fp32_lu"] = mixed_measure(ctx, one_run, cells, value, n, wr)
+    if not args.no_jac_reuse and args.advance_kernel == 1 and args.recipe == "reference":
+        ages = [int(a) for a in args.jac_ages.split(",") if a]
+        line["jacobian_reuse"] = jac_reuse_measure(ctx, one_run, cells, value, n, args.grid, wr,
+                                                   ages)
+    if not args.no_legacy and args.advance_kernel == 1:
+        line["legacy_local"] = legacy_measure(ctx, init, args.max_outer)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_measure(wr)
     if not args.no_cpu_baseline:
@@ -295,6 +309,67 @@ def mixed_measure(ctx, one_run, fp64_cells, fp64_value, n, wr):
         out[f"delta_{f}"] = int(got["stats"][f].sum()) - int(fp64_cells["stats"][f].sum())
     print(f"# mixed run: {el:.3f} s, fallbacks {fb}", file=sys.stderr, flush=True)
     return out
+
+
+def jac_reuse_measure(ctx, one_run, fp64_cells, fp64_value, n, grid, wr, ages):
+    """The same run with the Jacobian lagged up to `age` accepted steps (J
+    refreshed on rejection): speed, J evaluations, attempts, and the final
+    states against the exact-J run through the reference's comparator."""
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "exact.bin")
+    wr.write_final_states(fp64_cells, grid, path)
+    st0 = fp64_cells["stats"]
+    runs = []
+    try:
+        for age in ages:
+            ctx.set_jacobian_reuse(age)
+            info, el = one_run()
+            got = np.zeros(n, dtype=wr.CELL_DTYPE)
+            ctx.download(got)
+            cmp = wr.compare_final_states(got, grid, path)
+            st = got["stats"]
+            nd = [k for k, name in enumerate(wr.SPECIES_NAMES) if "D" not in name]
+            sp = max(rel_errors(got["current"]["xn"][:, k], fp64_cells["current"]["xn"][:, k])[0]
+                     for k in nd)
+            runs.append({
+                "age_max": age, "run_seconds": el, "value": info["cell_advances"] / el,
+                "speed_vs_exact_j": (info["cell_advances"] / el) / fp64_value,
+                "jacobian_calls": int(st["jacobian_calls"].sum()),
+                "jacobian_calls_exact_j": int(st0["jacobian_calls"].sum()),
+                "attempts": int(st["internal_steps"].sum()),
+                "attempts_exact_j": int(st0["internal_steps"].sum()),
+                "delta_accepted_steps": int(st["accepted_steps"].sum()) - int(st0["accepted_steps"].sum()),
+                "delta_rejected_steps": int(st["rejected_steps"].sum()) - int(st0["rejected_steps"].sum()),
+                "outer_steps": info["completed_global_steps"],
+                "reference_comparator_pass": cmp["pass"],
+                "max_non_deuterium_species_rel_err": sp,
+                "T_rel_err_max_median": list(rel_errors(got["current"]["T"],
+                                                        fp64_cells["current"]["T"]))})
+            print(f"# jacobian reuse {age}: {el:.3f} s, comparator {cmp['pass']}",
+                  file=sys.stderr, flush=True)
+    finally:
+        ctx.set_jacobian_reuse(1)
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"rule": "J at a burn's first attempt and after age_max accepted steps; "
+                    "refreshed when an attempt with a lagged J is rejected", "runs": runs}
+
+
+def legacy_measure(ctx, init, max_outer):
+    """The reference CPU build's driver policy (legacy-local: each cell steps
+    with its own free-fall dt) on the same cells: one run, timed."""
+    ctx.set_policy("legacy-local")
+    try:
+        ctx.upload(init)
+        t0 = time.perf_counter()
+        info = ctx.run(perturb=True, max_steps=max_outer)
+        el = time.perf_counter() - t0
+    finally:
+        ctx.set_policy("gridwide")
+    print(f"# legacy-local: {el:.3f} s, {info['completed_global_steps']} steps",
+          file=sys.stderr, flush=True)
+    return {"run_seconds": el, "value": info["cell_advances"] / el, "unit": "cell-advances/s",
+            "outer_steps": info["completed_global_steps"], "cell_advances": info["cell_advances"],
+            "status": info["status"], "stop_reason": info["stop_reason"]}
 
 
 def adjoint_measure(wr, grid=32, nsteps=20, seed=2, directions=8, delta=1e-6):

@@ -207,6 +207,19 @@ int wr_get_queue_launch(const wr_context *ctx, int *threads_per_block, int *bloc
  * in some component).  Mode 1 needs advance kernel 1 (runs otherwise fail
  * with WR_ERR_ARGUMENT). */
 int wr_set_lu_precision(wr_context *ctx, int mode, int sweeps);
+/* Jacobian reuse in the queue-fed kernel of subsequent gridwide steps (a study
+ * with no counterpart in the reference, which evaluates J at the head of every
+ * outer iteration of integrate and reuses it only on rejection,
+ * reproducer.cpp:7236-7260).  age_max = 1 (default): the reference.
+ * age_max = k (2..64): J is evaluated at a burn's first attempt and again once
+ * k steps have been accepted with it (the attempts in between evaluate f
+ * alone, so ROS2S runs as a W-method with a lagged matrix); an attempt
+ * rejected (error or singular matrix) with a J from an earlier state evaluates
+ * J at the current state before the retry.  The results differ from the
+ * reference's by what the controller makes of the lagged J.  Needs advance
+ * kernel 1 with FP64 factors and the gridwide policy (runs otherwise fail with
+ * WR_ERR_ARGUMENT). */
+int wr_set_jacobian_reuse(wr_context *ctx, int age_max);
 /* Fallback counts of the last wr_run in mode 1: attempts whose FP32 factors
  * were singular, stage solves whose refinement did not converge. */
 int wr_mixed_fallbacks(const wr_context *ctx, unsigned long long *lu_fallbacks,

@@ -333,6 +333,15 @@ void or_reset_mixed_fallbacks(void) {
     g_refine_fallbacks = 0;
 }
 
+/* Jacobian reuse (a study with no counterpart in the reference, which
+ * evaluates J at the head of every outer iteration of integrate and reuses it
+ * only on rejection, reproducer.cpp:7236-7260): J is evaluated at the first
+ * attempt of a burn and again once `age_max` steps have been accepted with it;
+ * a rejection of an attempt that used a J from an earlier state evaluates J
+ * at the current state before the retry.  age_max = 1 is the reference. */
+static int g_jac_age_max = 1;
+void or_set_jacobian_reuse(int age_max) { g_jac_age_max = age_max > 1 ? age_max : 1; }
+
 #define OR_REFINE_TOL 1.0e-6
 
 /* Stage solve of the FP32-factorisation form: x (in: b) from the FP32
@@ -505,6 +514,7 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
         h = ref_min(fabs(h), hmaxn);
         int reject = 0, last = 0, nsing = 0, n_step = 0, n_accept = 0;
         double hacc = 0.0, erracc = 1.0, hopt = h, x = t;
+        int jac_age = -1; /* accepted steps since J was evaluated; -1: none yet */
         for (;;) {
             if (n_step > max_steps) {
                 *t_end = x; *dt_end = h; result = OR_TOO_MANY_STEPS; goto done;
@@ -520,8 +530,11 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                 h = tout - x;
                 last = 1;
             }
-            hook_jac(y, fjac);
-            c.n_jac += 1;
+            if (jac_age < 0 || jac_age >= g_jac_age_max) {
+                hook_jac(y, fjac);
+                c.n_jac += 1;
+                jac_age = 0;
+            }
             for (;;) {
                 const double fac = 1.0 / (h * c_gamma);
                 for (int i = 0; i < NEQ; ++i) {
@@ -569,6 +582,11 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                     h *= 0.5;
                     reject = 1;
                     last = 0;
+                    if (jac_age > 0) { /* J from an earlier state: evaluate it here */
+                        hook_jac(y, fjac);
+                        c.n_jac += 1;
+                        jac_age = 0;
+                    }
                     continue;
                 }
                 c.n_decomp += 1;
@@ -630,6 +648,7 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                         t_nh += 1;
                     }
                     memcpy(y, ynew, sizeof(ynew));
+                    jac_age += 1;
                     x += h;
                     if (fabs(hnew) > hmaxn) hnew = hmaxn;
                     if (reject) hnew = ref_min(fabs(hnew), fabs(h));
@@ -641,6 +660,11 @@ int or_integrate(double *y, double tout, double dt0, int max_steps, int *counter
                 last = 0;
                 h = hnew;
                 if (n_accept >= 1) c.n_reject += 1;
+                if (jac_age > 0) { /* J from an earlier state: evaluate it here */
+                    hook_jac(y, fjac);
+                    c.n_jac += 1;
+                    jac_age = 0;
+                }
             }
         }
     }

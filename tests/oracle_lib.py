@@ -45,6 +45,7 @@ class _Cpu:
         if p == "or_":
             sig.update({"set_integrator_max_steps": (None, [_I]),
                         "set_lu_precision": (None, [_I, _I]),
+                        "set_jacobian_reuse": (None, [_I]),
                         "run_legacy": (_I, [_VP, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
                         "mixed_fallbacks": (None, [_VP, _VP]),
                         "reset_mixed_fallbacks": (None, []),

@@ -58,6 +58,7 @@ struct Ctl {
     int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
     int lu_sweeps;        // refinement sweeps per stage solve (FP32-factorisation form)
     int solo_lanes;       // queue positions reserved for block 0's first wave (0: none)
+    int jac_age_max;      // Jacobian reuse: accepted steps per J (1: the reference)
     unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
     unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
     // legacy-local policy: (failing cell << 32 | code) of the step's lowest
@@ -128,6 +129,8 @@ struct wr_context {
     // factorisation of the queue-fed kernel: 0 FP64, 1 FP32 + FP64 refinement
     int lu_mode = 0;
     int lu_sweeps = 2;
+    // Jacobian reuse of the queue-fed kernel: accepted steps per J (1: the reference)
+    int jac_age_max = 1;
     int integ_max_steps = wr::kMaxStepsDefault;
     // launch shape of the queue-fed kernel (wr_set_queue_launch): threads per
     // block, resident blocks per CU (0: the occupancy limit), solo-wave lanes

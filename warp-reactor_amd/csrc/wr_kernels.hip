@@ -730,6 +730,7 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.integ_max_steps = ctx->integ_max_steps;
     h.lu_sweeps = ctx->lu_sweeps;
     h.solo_lanes = ctx->q_solo;
+    h.jac_age_max = ctx->jac_age_max;
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
            "hipMemcpy(control to device)");
@@ -782,6 +783,10 @@ static int launch_advance(wr_context* ctx) {
     if (ctx->lu_mode != 0 && ctx->advance_kernel != 1)
         return fail(WR_ERR_ARGUMENT,
                     "FP32 factorisation (wr_set_lu_precision) needs the queue-fed advance kernel");
+    if (ctx->jac_age_max > 1 && (ctx->advance_kernel != 1 || ctx->lu_mode != 0))
+        return fail(WR_ERR_ARGUMENT,
+                    "Jacobian reuse (wr_set_jacobian_reuse) needs the queue-fed advance kernel "
+                    "with FP64 factors");
     if (ctx->advance_kernel == 1) {
         const int st = queue_grid(ctx);
         if (st != WR_SUCCESS) return st;
@@ -793,6 +798,10 @@ static int launch_advance(wr_context* ctx) {
         if (ctx->lu_mode)
             hipLaunchKernelGGL((k_advance_q<true, QPlainNet>), dim3(ctx->q_blocks), dim3(ctx->q_threads), 0,
                                ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
+        else if (ctx->jac_age_max > 1)
+            hipLaunchKernelGGL((k_advance_q<false, QPlainNet, false, true>), dim3(ctx->q_blocks),
+                               dim3(ctx->q_threads), 0, ctx->stream, ctx->cells, ctx->ctl, 0,
+                               QPlainNet{});
         else
             hipLaunchKernelGGL((k_advance_q<false, QPlainNet>), dim3(ctx->q_blocks), dim3(ctx->q_threads), 0,
                                ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
@@ -843,8 +852,8 @@ static int enqueue_steps(wr_context* ctx, int todo, bool timed) {
 // except the cell count and the advance kernel, so the graph is captured
 // again only when those change.  The graph holds kernel nodes only.
 static int chunk_graph(wr_context* ctx) {
-    const int kernel_key = ctx->advance_kernel + 2 * ctx->lu_mode +
-                           4 * (ctx->q_threads + 1024 * (ctx->q_per_cu + 64 * ctx->q_solo));
+    const int kernel_key = ctx->advance_kernel + 2 * ctx->lu_mode + 4 * (ctx->jac_age_max > 1) +
+                           8 * (ctx->q_threads + 1024 * (ctx->q_per_cu + 64 * ctx->q_solo));
     if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == kernel_key)
         return WR_SUCCESS;
     if (ctx->chunk_exec) {
@@ -968,9 +977,10 @@ int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_ste
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_run: null context");
     if (max_steps <= 0) max_steps = 1000;
     WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
-    if (ctx->policy == WR_POLICY_LEGACY_LOCAL && ctx->lu_mode != 0)
+    if (ctx->policy == WR_POLICY_LEGACY_LOCAL && (ctx->lu_mode != 0 || ctx->jac_age_max > 1))
         return fail(WR_ERR_ARGUMENT,
-                    "wr_run: the legacy-local policy factorises in FP64 (wr_set_lu_precision 0)");
+                    "wr_run: the legacy-local policy runs the reference integrator "
+                    "(wr_set_lu_precision 0, wr_set_jacobian_reuse 1)");
     int st = init_ctl(ctx, perturb, ctx->n > 0 ? max_steps : 0, completed_global_steps, grid_time);
     if (st != WR_SUCCESS) return st;
     ctx->adv_ms = 0.0;
@@ -1034,6 +1044,13 @@ int wr_set_policy(wr_context* ctx, int policy) {
     if (!ctx || (policy != WR_POLICY_GRIDWIDE && policy != WR_POLICY_LEGACY_LOCAL))
         return fail(WR_ERR_ARGUMENT, "wr_set_policy: policy must be 0 (gridwide) or 1 (legacy-local)");
     ctx->policy = policy;
+    return WR_SUCCESS;
+}
+
+int wr_set_jacobian_reuse(wr_context* ctx, int age_max) {
+    if (!ctx || age_max < 1 || age_max > 64)
+        return fail(WR_ERR_ARGUMENT, "wr_set_jacobian_reuse: age_max must be 1..64");
+    ctx->jac_age_max = age_max;
     return WR_SUCCESS;
 }
 

@@ -71,7 +71,13 @@ __device__ __forceinline__ void stage_solve(const ThreadWs& w, const float* lu32
 // 7646-7685): each cell perturbs itself and steps with its own free-fall dt;
 // a failure is recorded as (cell, code) by atomic minimum, so that the lowest
 // failing cell wins as in the reference's sequential cell loop.
-template <bool kMixed, class Net, bool kLocal = false>
+// kJacReuse: Jacobian reuse (no counterpart in the reference, which evaluates
+// J at the head of every outer iteration, 7236-7260): J is evaluated at a
+// burn's first attempt and again once Ctl::jac_age_max steps have been
+// accepted with it; an attempt rejected with a J from an earlier state
+// evaluates J at the current state before the retry.  Other attempts evaluate
+// f alone.
+template <bool kMixed, class Net, bool kLocal = false, bool kJacReuse = false>
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero, Net net) {
     using C = Ros2sCoef;
     if (ctl->done) return;
@@ -96,6 +102,8 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
     const int step = ctl->step;
     double tout = dt_grid;  // the burn's end time: dt_grid, or (kLocal) the cell's own dt
     const int max_steps = ctl->integ_max_steps;
+    const int jac_age_max = kJacReuse ? ctl->jac_age_max : 1;
+    int jac_age = -1;  // (kJacReuse) accepted steps since J was evaluated; -1: none yet
     int c = -1;
     double x = 0.0, h = 0.0, hopt = 0.0, hacc = 0.0, erracc = 1.0, rho = 0.0, e0 = 0.0;
     int nsing = 0, n_step = 0, n_accept = 0;
@@ -217,6 +225,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
             last = false;
             top = true;
             cnt = Counters{0, 0, 0, 0, 0, 0, 0};
+            jac_age = -1;
         }
         int result = 0;  // 0: the cell goes on
         if (top) {
@@ -233,8 +242,19 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                     h = tout - x;
                     last = true;
                 }
-                net.jac_rhs(w);
-                cnt.n_jac += 1;
+                if (!kJacReuse || jac_age < 0 || jac_age >= jac_age_max) {
+                    net.jac_rhs(w);
+                    cnt.n_jac += 1;
+                    jac_age = 0;
+                } else {  // f(y) alone, through the stage evaluation's slots
+#pragma unroll
+                    for (int i = 0; i < kNeqs; ++i)
+                        WR_AT(w.vec, kVecYnew + i, st) = WR_AT(w.vec, kVecY + i, st);
+                    net.rhs(w);
+#pragma unroll
+                    for (int i = 0; i < kNeqs; ++i)
+                        WR_AT(w.vec, kVecF0 + i, st) = WR_AT(w.vec, kVecDy + i, st);
+                }
                 top = false;
             }
         }
@@ -263,6 +283,11 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                     h *= 0.5;
                     reject = true;
                     last = false;
+                    if (kJacReuse && jac_age > 0) {  // J from an earlier state
+                        net.jac_rhs(w);
+                        cnt.n_jac += 1;
+                        jac_age = 0;
+                    }
                 }
             } else {
                 cnt.n_decomp += 1;
@@ -332,6 +357,7 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                     for (int i = 0; i < kNeqs; ++i)
                         WR_AT(w.vec, kVecY + i, st) = WR_AT(w.vec, kVecYnew + i, st);
                     net.accepted(c, h);
+                    jac_age += 1;
                     x += h;
                     if (fabs(hnew) > tout) hnew = tout;
                     if (reject) hnew = rmin(fabs(hnew), fabs(h));
@@ -343,6 +369,11 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                     last = false;
                     h = hnew;
                     if (n_accept >= 1) cnt.n_reject += 1;
+                    if (kJacReuse && jac_age > 0) {  // J from an earlier state
+                        net.jac_rhs(w);
+                        cnt.n_jac += 1;
+                        jac_age = 0;
+                    }
                 }
             }
         }

@@ -80,6 +80,7 @@ _SIGS = {
     "wr_dt_history": (_I, [_VP, _VP, _I, ctypes.POINTER(_I)]),
     "wr_set_advance_kernel": (_I, [_VP, _I]),
     "wr_set_policy": (_I, [_VP, _I]),
+    "wr_set_jacobian_reuse": (_I, [_VP, _I]),
     "wr_set_timing": (_I, [_VP, _I]),
     "wr_set_integrator_max_steps": (_I, [_VP, _I]),
     "wr_set_graph": (_I, [_VP, _I]),
@@ -266,6 +267,11 @@ class Context:
         own free-fall dt)."""
         code = {"gridwide": 0, "legacy-local": 1}.get(policy, policy)
         _check(self._lib.wr_set_policy(self._h, int(code)), "wr_set_policy")
+
+    def set_jacobian_reuse(self, age_max):
+        """Accepted steps per Jacobian in the queue-fed kernel (1: the reference;
+        k > 1: J lagged up to k accepted steps, refreshed on rejection)."""
+        _check(self._lib.wr_set_jacobian_reuse(self._h, int(age_max)), "wr_set_jacobian_reuse")
 
     def set_advance_kernel(self, kernel):
         """1: queue-fed persistent kernel (default); 0: one thread per cell."""
