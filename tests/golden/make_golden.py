@@ -18,6 +18,8 @@ oracle/ref_harness.cpp.  Inputs are seeded.  Files (numpy .npz, no pickles):
                   semantic cases, final cells of the grid-1 run and of small
                   gridwide runs (216-byte records)
   final_states_grid2.bin   packed file written by the reference writer
+  final_states_grid2_legacy_local.bin   the same for the reference CPU build's
+                  legacy-local loop (reproducer.cpp:8628-8648) on 8 cells
 """
 import ctypes
 import hashlib
@@ -141,6 +143,15 @@ def main():
                                                ctypes.c_char_p]
     ok = ref.lib.ref_write_final_states(g8.ctypes.data_as(ctypes.c_void_p), 8, 2, path.encode())
     assert ok == 1
+    legacy = ref.make_cells(8, CELL_DTYPE)
+    r, done = ref.run_legacy(legacy, True)
+    assert r == 1
+    path = os.path.join(HERE, "final_states_grid2_legacy_local.bin")
+    if os.path.exists(path):
+        os.remove(path)
+    ok = ref.lib.ref_write_final_states(legacy.ctypes.data_as(ctypes.c_void_p), 8, 2, path.encode())
+    assert ok == 1
+    meta["grid2_legacy_local_steps"] = int(done)
     meta["files"] = {n: hashlib.sha256(open(os.path.join(HERE, n), "rb").read()).hexdigest()
                      for n in sorted(os.listdir(HERE)) if n.endswith((".npz", ".bin"))}
     with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:

@@ -31,3 +31,11 @@ def test_bad_arguments_fail_with_usage(args, tmp_path):
     assert "usage: " in r.stderr
     assert r.stdout == ""
     assert not os.listdir(tmp_path)  # nothing written
+
+
+def test_driver_policy_option(tmp_path):
+    assert "--driver-policy gridwide-independent|legacy-local" in run("--help").stderr
+    for args in (["--driver-policy"], ["--driver-policy", "gridwide-cta32"]):
+        r = run(*args, cwd=tmp_path)
+        assert r.returncode == 1 and "usage: " in r.stderr and r.stdout == ""
+    assert not os.listdir(tmp_path)

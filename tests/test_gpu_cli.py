@@ -98,3 +98,23 @@ def test_cli_rotates_and_reports_failures(tmp_path, wr, oracle_grid2):
     assert (tmp_path / "final_states_grid2_hip.bin").exists()
     csv = (tmp_path / "final_state_comparison_failures_hip.csv").read_text().splitlines()
     assert len(csv) >= 2 and "cell" in csv[0]
+
+
+def test_cli_legacy_local_policy(tmp_path, wr, oracle_wr):
+    """--driver-policy legacy-local: the reference CPU build's loop on the
+    device; its summary line, the oracle's step count, the file bytes of the
+    oracle's grid and a PASS against the reference CPU build's own file
+    (tests/golden/final_states_grid2_legacy_local.bin)."""
+    gold = os.path.join(REPO, "tests", "golden", "final_states_grid2_legacy_local.bin")
+    ocells = oracle_wr.make_cells(8, wr.CELL_DTYPE)
+    orr, odone, ofailed = oracle_wr.run_legacy(ocells, True)
+    assert orr == 1 and ofailed == -1
+    r = run(["--grid", "2", "--driver-policy", "legacy-local", "--compare-final-state", gold],
+            tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[3:5] == ["driver policy: legacy-local", f"completed global collapse steps: {odone}"]
+    assert any(l.startswith("final-state comparison: PASS (max deuterium-bearing") for l in lines)
+    assert lines[-1] == "PASS"
+    data = (tmp_path / "final_states_grid2_hip_legacy_local.bin").read_bytes()
+    assert data == wr.pack_final_states(ocells, 2).tobytes()

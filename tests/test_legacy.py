@@ -22,3 +22,19 @@ def test_legacy_driver_matches_reference(oracle, ref, wr, n, perturb):
     assert _equal(cells, rcells)
     if n == 1:
         assert done == 438  # the grid-1 known answer (legacy-local = gridwide for one cell)
+
+
+def test_legacy_driver_golden_file(oracle, wr):
+    """The committed fixture final_states_grid2_legacy_local.bin was written by
+    the reference's writer from its legacy-local loop on 8 cells
+    (tests/golden/make_golden.py); the oracle's restatement packs to the same
+    bytes."""
+    import json
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    cells = oracle.make_cells(8, wr.CELL_DTYPE)
+    r, done, failed = oracle.run_legacy(cells, True)
+    meta = json.load(open(os.path.join(gold, "MANIFEST.json")))
+    assert r == 1 and failed == -1 and done == meta["grid2_legacy_local_steps"]
+    data = open(os.path.join(gold, "final_states_grid2_legacy_local.bin"), "rb").read()
+    assert data == wr.pack_final_states(cells, 2).tobytes()
