@@ -467,16 +467,19 @@ __device__ __forceinline__ int lu_left_t(const double* __restrict__ J, R* __rest
 #pragma unroll
         for (int k = 0; k < kNeqs - 1; ++k) {
             if (k < j && !((sing >> k) & 1u)) {
+                // multipliers of column k first: all loads of the column are in
+                // flight together instead of one load and wait per update
+                R m[kNeqs];
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i)
+                    if (i > k)
+                        m[i] = (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
+                if (k >= kCache) __builtin_amdgcn_sched_barrier(0);
                 swap_rows(col, k, piv_get(piv, k));
                 const R t = col[k];
 #pragma unroll
-                for (int i = 0; i < kNeqs; ++i) {
-                    if (i > k) {
-                        const R m =
-                            (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
-                        col[i] += t * m;
-                    }
-                }
+                for (int i = 0; i < kNeqs; ++i)
+                    if (i > k) col[i] += t * m[i];
             }
         }
         if (j < kNeqs - 1) {
@@ -528,22 +531,35 @@ __device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& 
 template <typename R>
 __device__ __forceinline__ void lu_solve_t(const R* __restrict__ LU, int st, uint64_t piv,
                                            R (&x)[kNeqs]) {
+    // Each column of the factors is loaded whole before it is used (the
+    // scheduling barrier keeps the loads together), so its loads overlap
+    // instead of each update waiting for its own load.
 #pragma unroll
     for (int k = 0; k < kNeqs - 1; ++k) {
+        R m[kNeqs];
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j)
+            if (j > k) m[j] = WR_AT(LU, j * kNeqs + k, st);
+        __builtin_amdgcn_sched_barrier(0);
         swap_rows(x, k, piv_get(piv, k));
         const R t = x[k];
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j > k) x[j] += t * WR_AT(LU, j * kNeqs + k, st);
+            if (j > k) x[j] += t * m[j];
     }
 #pragma unroll
     for (int kb = 0; kb < kNeqs; ++kb) {
         const int k = kNeqs - 1 - kb;
-        x[k] /= WR_AT(LU, k * kNeqs + k, st);
+        R m[kNeqs];
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j)
+            if (j <= k) m[j] = WR_AT(LU, j * kNeqs + k, st);
+        __builtin_amdgcn_sched_barrier(0);
+        x[k] /= m[k];
         const R t = -x[k];
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j < k) x[j] += t * WR_AT(LU, j * kNeqs + k, st);
+            if (j < k) x[j] += t * m[j];
     }
 }
 
