@@ -46,6 +46,14 @@ constexpr int kSoloLanes = 64;  // default of Ctl::solo_lanes
 // equation, error norm and controller stay FP64.
 // Stage solve of the queue-fed kernel: FP64 factors, or (kMixed) FP32 factors
 // with refinement and the FP64 factors from the first non-convergence on.
+// One stage vector of the lane workspace into registers, its loads issued
+// together (the scheduling barrier keeps them ahead of their uses).
+__device__ __forceinline__ void load_vec(const ThreadWs& w, int off, double (&out)[kNeqs]) {
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) out[i] = WR_AT(w.vec, off + i, w.stride);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 template <bool kMixed>
 __device__ __forceinline__ void stage_solve(const ThreadWs& w, const float* lu32, uint64_t piv32,
                                             const int (&ks)[kNeqs], double fac, int sweeps,
@@ -291,43 +299,52 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                 }
             } else {
                 cnt.n_decomp += 1;
-                double v[kNeqs];
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i) v[i] = WR_AT(w.vec, kVecF0 + i, st);
+                // Stage vectors are read from the workspace in whole vectors
+                // (load_vec) before they are used, so their loads overlap.
+                double v[kNeqs], a[kNeqs], b[kNeqs];
+                load_vec(w, kVecF0, v);
                 stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak1
                 cnt.n_solve += 1;
+                load_vec(w, kVecY, a);
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
                     WR_AT(w.vec, kVecAk1 + i, st) = v[i];
-                    WR_AT(w.vec, kVecYnew + i, st) = WR_AT(w.vec, kVecY + i, st) + C::a21 * v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) = a[i] + C::a21 * v[i];
                     WR_AT(w.vec, kVecAk2 + i, st) = (C::c21 / h) * v[i];
                 }
                 net.rhs(w);
+                load_vec(w, kVecAk2, a);
+                load_vec(w, kVecDy, b);
 #pragma unroll
-                for (int i = 0; i < kNeqs; ++i)
-                    v[i] = WR_AT(w.vec, kVecAk2 + i, st) + WR_AT(w.vec, kVecDy + i, st);
+                for (int i = 0; i < kNeqs; ++i) v[i] = a[i] + b[i];
                 stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak2
                 cnt.n_solve += 1;
+                load_vec(w, kVecAk1, a);
+                load_vec(w, kVecY, b);
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
-                    const double a1 = WR_AT(w.vec, kVecAk1 + i, st);
+                    const double a1 = a[i];
                     WR_AT(w.vec, kVecAk2 + i, st) = v[i];
-                    WR_AT(w.vec, kVecYnew + i, st) =
-                        WR_AT(w.vec, kVecY + i, st) + C::a31 * a1 + C::a32 * v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) = b[i] + C::a31 * a1 + C::a32 * v[i];
                     WR_AT(w.vec, kVecAk3 + i, st) = (C::c31 * a1 + C::c32 * v[i]) / h;
                 }
                 net.rhs(w);
+                load_vec(w, kVecAk3, a);
+                load_vec(w, kVecDy, b);
 #pragma unroll
-                for (int i = 0; i < kNeqs; ++i)
-                    v[i] = WR_AT(w.vec, kVecAk3 + i, st) + WR_AT(w.vec, kVecDy + i, st);
+                for (int i = 0; i < kNeqs; ++i) v[i] = a[i] + b[i];
                 stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak3
                 cnt.n_solve += 1;
                 double err = 0.0;
+                load_vec(w, kVecAk1, a);
+                load_vec(w, kVecAk2, b);
+                double yv[kNeqs];
+                load_vec(w, kVecY, yv);
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) {
-                    const double a1 = WR_AT(w.vec, kVecAk1 + i, st);
-                    const double a2 = WR_AT(w.vec, kVecAk2 + i, st);
-                    const double yi = WR_AT(w.vec, kVecY + i, st);
+                    const double a1 = a[i];
+                    const double a2 = b[i];
+                    const double yi = yv[i];
                     const double yn = yi + C::b1 * a1 + C::b2 * a2 + C::b3 * v[i];
                     WR_AT(w.vec, kVecYnew + i, st) = yn;
                     const double wv = C::e1 * a1 + C::e2 * a2 + C::e3 * v[i];
@@ -353,9 +370,9 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                     }
                     hacc = h;
                     erracc = rmax(1.0e-2, err);
+                    load_vec(w, kVecYnew, a);
 #pragma unroll
-                    for (int i = 0; i < kNeqs; ++i)
-                        WR_AT(w.vec, kVecY + i, st) = WR_AT(w.vec, kVecYnew + i, st);
+                    for (int i = 0; i < kNeqs; ++i) WR_AT(w.vec, kVecY + i, st) = a[i];
                     net.accepted(c, h);
                     jac_age += 1;
                     x += h;
