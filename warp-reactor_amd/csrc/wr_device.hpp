@@ -464,22 +464,40 @@ __device__ __forceinline__ int lu_left_t(const double* __restrict__ J, R* __rest
             const double eij = (i == j) ? v + fac : v;
             col[i] = ks ? static_cast<R>(ldexp(eij, ks[j] - ks[i])) : static_cast<R>(eij);
         }
+        // updates from columns 0..j-1 in order: the first kCache from registers,
+        // the rest from LU with the multipliers of 4 columns loaded in one batch
+        // (the scheduling barrier keeps the loads together, ahead of their uses)
 #pragma unroll
-        for (int k = 0; k < kNeqs - 1; ++k) {
+        for (int k = 0; k < kCache; ++k) {
             if (k < j && !((sing >> k) & 1u)) {
-                // multipliers of column k first: all loads of the column are in
-                // flight together instead of one load and wait per update
-                R m[kNeqs];
-#pragma unroll
-                for (int i = 0; i < kNeqs; ++i)
-                    if (i > k)
-                        m[i] = (k < kCache) ? mc[k < kCache ? k : 0][i] : WR_AT(LU, i * kNeqs + k, st);
-                if (k >= kCache) __builtin_amdgcn_sched_barrier(0);
                 swap_rows(col, k, piv_get(piv, k));
                 const R t = col[k];
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i)
-                    if (i > k) col[i] += t * m[i];
+                    if (i > k) col[i] += t * mc[k][i];
+            }
+        }
+#pragma unroll
+        for (int kb = kCache; kb < kNeqs - 1; kb += 4) {
+            if (kb < j) {
+                R mm[4][kNeqs];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int i = 0; i < kNeqs; ++i)
+                        if (kb + q < j && i > kb + q) mm[q][i] = WR_AT(LU, i * kNeqs + kb + q, st);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = kb + q;
+                    if (k < j && !((sing >> k) & 1u)) {
+                        swap_rows(col, k, piv_get(piv, k));
+                        const R t = col[k];
+#pragma unroll
+                        for (int i = 0; i < kNeqs; ++i)
+                            if (i > k) col[i] += t * mm[q][i];
+                    }
+                }
             }
         }
         if (j < kNeqs - 1) {
@@ -527,40 +545,65 @@ __device__ __forceinline__ int lu_left(const ThreadWs& w, double fac, uint64_t& 
     return lu_left_t<double>(w.J, w.LU, w.stride, fac, piv);
 }
 
-// lu_solve (reproducer.cpp:154-181) from the workspace factors, x in registers.
-template <typename R>
-__device__ __forceinline__ void lu_solve_t(const R* __restrict__ LU, int st, uint64_t piv,
-                                           R (&x)[kNeqs]) {
-    // Each column of the factors is loaded whole before it is used (the
-    // scheduling barrier keeps the loads together), so its loads overlap
-    // instead of each update waiting for its own load.
+// Forward elimination over columns K0..K1-1: their multipliers are loaded
+// in one batch (the scheduling barrier keeps the loads together, ahead of
+// their uses), so the loads overlap instead of each update waiting for its
+// own load.
+template <typename R, int K0, int K1>
+__device__ __forceinline__ void solve_fwd_cols(const R* __restrict__ LU, int st, uint64_t piv,
+                                               R (&x)[kNeqs]) {
+    R m[K1 - K0][kNeqs];
 #pragma unroll
-    for (int k = 0; k < kNeqs - 1; ++k) {
-        R m[kNeqs];
+    for (int k = K0; k < K1; ++k)
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j > k) m[j] = WR_AT(LU, j * kNeqs + k, st);
-        __builtin_amdgcn_sched_barrier(0);
+            if (j > k) m[k - K0][j] = WR_AT(LU, j * kNeqs + k, st);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = K0; k < K1; ++k) {
         swap_rows(x, k, piv_get(piv, k));
         const R t = x[k];
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j > k) x[j] += t * m[j];
+            if (j > k) x[j] += t * m[k - K0][j];
     }
+}
+
+// Back substitution over columns K1-1 down to K0, their upper parts loaded
+// in one batch.
+template <typename R, int K0, int K1>
+__device__ __forceinline__ void solve_bwd_cols(const R* __restrict__ LU, int st, R (&x)[kNeqs]) {
+    R m[K1 - K0][kNeqs];
 #pragma unroll
-    for (int kb = 0; kb < kNeqs; ++kb) {
-        const int k = kNeqs - 1 - kb;
-        R m[kNeqs];
+    for (int k = K0; k < K1; ++k)
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j <= k) m[j] = WR_AT(LU, j * kNeqs + k, st);
-        __builtin_amdgcn_sched_barrier(0);
-        x[k] /= m[k];
+            if (j <= k) m[k - K0][j] = WR_AT(LU, j * kNeqs + k, st);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = K1 - 1; k >= K0; --k) {
+        x[k] /= m[k - K0][k];
         const R t = -x[k];
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j < k) x[j] += t * m[j];
+            if (j < k) x[j] += t * m[k - K0][j];
     }
+}
+
+// lu_solve (reproducer.cpp:154-181) from the workspace factors, x in registers:
+// the same operations in the same order, the factors read in batches of
+// 21-50 values (forward columns 0-3, 4-7, 8-13; backward 14-12, 11-8, 7-4, 3-0).
+template <typename R>
+__device__ __forceinline__ void lu_solve_t(const R* __restrict__ LU, int st, uint64_t piv,
+                                           R (&x)[kNeqs]) {
+    static_assert(kNeqs == 15, "batch boundaries are chosen for 15 unknowns");
+    solve_fwd_cols<R, 0, 4>(LU, st, piv, x);
+    solve_fwd_cols<R, 4, 8>(LU, st, piv, x);
+    solve_fwd_cols<R, 8, 14>(LU, st, piv, x);
+    solve_bwd_cols<R, 12, 15>(LU, st, x);
+    solve_bwd_cols<R, 8, 12>(LU, st, x);
+    solve_bwd_cols<R, 4, 8>(LU, st, x);
+    solve_bwd_cols<R, 0, 4>(LU, st, x);
 }
 
 __device__ __forceinline__ void lu_solve_ws(const ThreadWs& w, uint64_t piv, double (&x)[kNeqs]) {
