@@ -178,28 +178,65 @@ __device__ __forceinline__ int adj_substep(double (&y)[kNeqs], double h, const d
     return kSuccess;
 }
 
-// E^T x = b from the LINPACK factors (dgesl with job != 0): U^T, then each L_k^T
-// and interchange in reverse elimination order.
-__device__ __forceinline__ void lu_solve_t_ws(const double* __restrict__ LU, int st, uint64_t piv,
-                                              double (&x)[kNeqs]) {
+// U^T over columns K0..K1-1 in order, their upper parts (with the diagonal)
+// loaded in one batch ahead of their uses (the scheduling barrier keeps the
+// loads together, so they overlap).
+template <int K0, int K1>
+__device__ __forceinline__ void tsolve_upper_cols(const double* __restrict__ LU, int st,
+                                                  double (&x)[kNeqs]) {
+    double m[K1 - K0][kNeqs];
 #pragma unroll
-    for (int k = 0; k < kNeqs; ++k) {
+    for (int k = K0; k < K1; ++k)
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j)
+            if (j <= k) m[k - K0][j] = WR_AT(LU, j * kNeqs + k, st);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = K0; k < K1; ++k) {
         double t = 0.0;
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j < k) t += WR_AT(LU, j * kNeqs + k, st) * x[j];
-        x[k] = (x[k] - t) / WR_AT(LU, k * kNeqs + k, st);
+            if (j < k) t += m[k - K0][j] * x[j];
+        x[k] = (x[k] - t) / m[k - K0][k];
     }
+}
+
+// L_k^T and the interchanges over columns K1-1 down to K0, their multipliers
+// loaded in one batch.
+template <int K0, int K1>
+__device__ __forceinline__ void tsolve_lower_cols(const double* __restrict__ LU, int st,
+                                                  uint64_t piv, double (&x)[kNeqs]) {
+    double m[K1 - K0][kNeqs];
 #pragma unroll
-    for (int kb = 0; kb < kNeqs - 1; ++kb) {
-        const int k = kNeqs - 2 - kb;
+    for (int k = K0; k < K1; ++k)
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j)
+            if (j > k) m[k - K0][j] = WR_AT(LU, j * kNeqs + k, st);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = K1 - 1; k >= K0; --k) {
         double t = 0.0;
 #pragma unroll
         for (int j = 0; j < kNeqs; ++j)
-            if (j > k) t += WR_AT(LU, j * kNeqs + k, st) * x[j];
+            if (j > k) t += m[k - K0][j] * x[j];
         x[k] += t;
         swap_rows(x, k, piv_get(piv, k));
     }
+}
+
+// E^T x = b from the LINPACK factors (dgesl with job != 0): U^T, then each L_k^T
+// and interchange in reverse elimination order; the factors are read in
+// batches of 10-50 values.
+__device__ __forceinline__ void lu_solve_t_ws(const double* __restrict__ LU, int st, uint64_t piv,
+                                              double (&x)[kNeqs]) {
+    static_assert(kNeqs == 15, "batch boundaries are chosen for 15 unknowns");
+    tsolve_upper_cols<0, 4>(LU, st, x);
+    tsolve_upper_cols<4, 8>(LU, st, x);
+    tsolve_upper_cols<8, 12>(LU, st, x);
+    tsolve_upper_cols<12, 15>(LU, st, x);
+    tsolve_lower_cols<8, 14>(LU, st, piv, x);
+    tsolve_lower_cols<4, 8>(LU, st, piv, x);
+    tsolve_lower_cols<0, 4>(LU, st, piv, x);
 }
 
 // Stored pieces of one accepted step (kept by k_adj_steps for the reverse
@@ -240,10 +277,14 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
             u[i] = stage == 1 ? y[i] : slot[(uf + i) * sst];
         }
         lu_solve_t_ws(LU, sst, piv, rb);
+        double kv[kNeqs];  // k_stage, loaded once for the 225 products below
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j) kv[j] = slot[(kf + j) * sst];
+        __builtin_amdgcn_sched_barrier(0);
         if (stage == 3) {
             for (int i = 0; i < kNeqs; ++i)
 #pragma unroll
-                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] = rb[i] * slot[(kf + j) * sst];
+                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] = rb[i] * kv[j];
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i) {
                 kb1[i] += (C::c31 / h) * rb[i];
@@ -252,7 +293,7 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
         } else {
             for (int i = 0; i < kNeqs; ++i)
 #pragma unroll
-                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] += rb[i] * slot[(kf + j) * sst];
+                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] += rb[i] * kv[j];
             if (stage == 2) {
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) kb1[i] += (C::c21 / h) * rb[i];
