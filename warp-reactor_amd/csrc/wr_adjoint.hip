@@ -248,7 +248,8 @@ constexpr int kSY = 0, kSLU = 15, kSK1 = 240, kSK2 = 255, kSK3 = 270, kSU2 = 285
 // W, the y-adjoint output of the generated code, dL/dM of one f adjoint and the
 // cell's accumulated dL/dM, and the staging slots.
 constexpr int kPW = 0, kPUB = 225, kPMT = 240, kPMB = 240 + kR, kPSTG = 240 + 2 * kR;
-constexpr int kPrivRev = kPSTG + AD_NSLOT;
+constexpr int kPRB = kPSTG + AD_NSLOT;  // r-bar of stages 3, 2, 1 (15 each)
+constexpr int kPrivRev = kPRB + 3 * kNeqs;
 
 // adjoint of one accepted step: yb in/out.  LU, k, u from the stored slot
 // (element e at slot[e * sst]); W, staging and dL/dM in private memory p.
@@ -269,7 +270,6 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
 #pragma unroll 1
     for (int stage = 3; stage >= 1; --stage) {
         double rb[kNeqs], u[kNeqs];
-        const int kf = stage == 3 ? kSK3 : stage == 2 ? kSK2 : kSK1;
         const int uf = stage == 3 ? kSU3 : kSU2;
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) {
@@ -277,23 +277,16 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
             u[i] = stage == 1 ? y[i] : slot[(uf + i) * sst];
         }
         lu_solve_t_ws(LU, sst, piv, rb);
-        double kv[kNeqs];  // k_stage, loaded once for the 225 products below
+        // r-bar of the stage, for W = sum_s r-bar_s k_s^T after the loop
 #pragma unroll
-        for (int j = 0; j < kNeqs; ++j) kv[j] = slot[(kf + j) * sst];
-        __builtin_amdgcn_sched_barrier(0);
+        for (int i = 0; i < kNeqs; ++i) p[kPRB + (3 - stage) * kNeqs + i] = rb[i];
         if (stage == 3) {
-            for (int i = 0; i < kNeqs; ++i)
-#pragma unroll
-                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] = rb[i] * kv[j];
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i) {
                 kb1[i] += (C::c31 / h) * rb[i];
                 kb2[i] += (C::c32 / h) * rb[i];
             }
         } else {
-            for (int i = 0; i < kNeqs; ++i)
-#pragma unroll
-                for (int j = 0; j < kNeqs; ++j) p[kPW + i * kNeqs + j] += rb[i] * kv[j];
             if (stage == 2) {
 #pragma unroll
                 for (int i = 0; i < kNeqs; ++i) kb1[i] += (C::c21 / h) * rb[i];
@@ -310,6 +303,28 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
                 kb2[i] += C::a32 * ub;
             } else if (stage == 2) {
                 kb1[i] += C::a21 * ub;
+            }
+        }
+    }
+    // W = r-bar_3 k3^T + r-bar_2 k2^T + r-bar_1 k1^T, each element summed in that
+    // order (stage 3 first, as the stages are reversed), written once
+    {
+        double k3[kNeqs], k2[kNeqs], k1[kNeqs];
+#pragma unroll
+        for (int j = 0; j < kNeqs; ++j) {
+            k3[j] = slot[(kSK3 + j) * sst];
+            k2[j] = slot[(kSK2 + j) * sst];
+            k1[j] = slot[(kSK1 + j) * sst];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        for (int i = 0; i < kNeqs; ++i) {
+            const double r3 = p[kPRB + i], r2 = p[kPRB + kNeqs + i], r1 = p[kPRB + 2 * kNeqs + i];
+#pragma unroll
+            for (int j = 0; j < kNeqs; ++j) {
+                double t = r3 * k3[j];
+                t += r2 * k2[j];
+                t += r1 * k1[j];
+                p[kPW + i * kNeqs + j] = t;
             }
         }
     }
