@@ -293,7 +293,21 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
             }
         }
         ad_f_vjp_st(u, 1, M, 1, rb, 1, p + kPSTG, 1, p + kPUB, p + kPMT);
-        for (int r = 0; r < kR; ++r) p[kPMB + r] += p[kPMT + r];
+        // dL/dM of the call added to the cell's sum, 10 multipliers per batch of loads
+#pragma unroll
+        for (int r0 = 0; r0 < kR; r0 += 10) {
+            double mt[10], mb[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q)
+                if (r0 + q < kR) {
+                    mt[q] = p[kPMT + r0 + q];
+                    mb[q] = p[kPMB + r0 + q];
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 10; ++q)
+                if (r0 + q < kR) p[kPMB + r0 + q] = mb[q] + mt[q];
+        }
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) {
             const double ub = p[kPUB + i];
