@@ -303,39 +303,38 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                 // (load_vec) before they are used, so their loads overlap.
                 double v[kNeqs], a[kNeqs], b[kNeqs];
                 load_vec(w, kVecF0, v);
-                // stages 1-3 through one loop body: one instance each of the
-                // stage solve and of the generated f in the kernel's code
-#pragma unroll 1
-                for (int sg = 1; sg <= 3; ++sg) {
-                    if (sg > 1) {
-                        if (sg == 2) {
-                            load_vec(w, kVecY, a);
+                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak1
+                cnt.n_solve += 1;
+                load_vec(w, kVecY, a);
 #pragma unroll
-                            for (int i = 0; i < kNeqs; ++i) {
-                                WR_AT(w.vec, kVecAk1 + i, st) = v[i];
-                                WR_AT(w.vec, kVecYnew + i, st) = a[i] + C::a21 * v[i];
-                                WR_AT(w.vec, kVecAk2 + i, st) = (C::c21 / h) * v[i];
-                            }
-                        } else {
-                            load_vec(w, kVecAk1, a);
-                            load_vec(w, kVecY, b);
-#pragma unroll
-                            for (int i = 0; i < kNeqs; ++i) {
-                                const double a1 = a[i];
-                                WR_AT(w.vec, kVecAk2 + i, st) = v[i];
-                                WR_AT(w.vec, kVecYnew + i, st) = b[i] + C::a31 * a1 + C::a32 * v[i];
-                                WR_AT(w.vec, kVecAk3 + i, st) = (C::c31 * a1 + C::c32 * v[i]) / h;
-                            }
-                        }
-                        net.rhs(w);
-                        load_vec(w, sg == 2 ? kVecAk2 : kVecAk3, a);
-                        load_vec(w, kVecDy, b);
-#pragma unroll
-                        for (int i = 0; i < kNeqs; ++i) v[i] = a[i] + b[i];
-                    }
-                    stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);
-                    cnt.n_solve += 1;
+                for (int i = 0; i < kNeqs; ++i) {
+                    WR_AT(w.vec, kVecAk1 + i, st) = v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) = a[i] + C::a21 * v[i];
+                    WR_AT(w.vec, kVecAk2 + i, st) = (C::c21 / h) * v[i];
                 }
+                net.rhs(w);
+                load_vec(w, kVecAk2, a);
+                load_vec(w, kVecDy, b);
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) v[i] = a[i] + b[i];
+                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak2
+                cnt.n_solve += 1;
+                load_vec(w, kVecAk1, a);
+                load_vec(w, kVecY, b);
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) {
+                    const double a1 = a[i];
+                    WR_AT(w.vec, kVecAk2 + i, st) = v[i];
+                    WR_AT(w.vec, kVecYnew + i, st) = b[i] + C::a31 * a1 + C::a32 * v[i];
+                    WR_AT(w.vec, kVecAk3 + i, st) = (C::c31 * a1 + C::c32 * v[i]) / h;
+                }
+                net.rhs(w);
+                load_vec(w, kVecAk3, a);
+                load_vec(w, kVecDy, b);
+#pragma unroll
+                for (int i = 0; i < kNeqs; ++i) v[i] = a[i] + b[i];
+                stage_solve<kMixed>(w, lu32, piv32, ks, fac, sweeps, ctl, use64, piv, v);  // ak3
+                cnt.n_solve += 1;
                 double err = 0.0;
                 load_vec(w, kVecAk1, a);
                 load_vec(w, kVecAk2, b);
