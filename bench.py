@@ -13,10 +13,14 @@ integrated cells / wall time of the K timed runs.  `roofline` is the FP64
 roofline of the advance kernel: algorithmic FP64 operations (static counts of
 the generated network and of the LU/solve/stage arithmetic, times the ROS2S
 counters of the run) per advance launch / its mean duration measured with HIP
-events on the launch stream.  `cpu_baseline` times the reference's own CPU
-build (oracle/_ref, single-threaded) on a bounded sample of the same
-workload.  `roofline.traffic` is the HBM bytes per advance dispatch from
-the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated).
+events on the launch stream.  `cpu_baseline` times the reference's own
+compiled CPU code (oracle/_ref/libwr_ref.so) on a bounded, strided sample of
+the same workload -- cells of the run replayed through its dt_grid sequence --
+on 1 thread and on all the host threads the box gives the process.
+`roofline.traffic` is the HBM bytes per advance dispatch measured in the same
+run of bench.py: two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over the
+same workload and launch shape, each calibrated by 1 GiB device copies in the
+same process (warp-reactor_amd/prof/traffic_run.py).
 `plain_fp64` times the same run with the thread-per-cell kernel (no queue,
 network left to the compiler's register allocation) and checks that the
 tuned default produces bit-identical final states.
@@ -37,19 +41,62 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X vector FP64 (= FP64 matrix), dense
-TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "k_advance_q_64cube_pmc_traffic.txt")
+PROF_DIR = os.path.join(REPO, "warp-reactor_amd", "prof")
 
 
-def measured_traffic(kernel, grid):
-    """HBM bytes per dispatch of the advance kernel from the committed counter
-    summary (64^3 default run only), or None."""
-    if kernel != "k_advance_q" or grid != 64 or not os.path.exists(TRAFFIC_SUMMARY):
+def live_traffic(grid, max_outer, shape, kernel_name="k_advance_q", timeout=300):
+    """HBM bytes per dispatch of the advance kernel in this build, measured
+    now: two runs of prof/traffic_run.py (same workload and launch shape)
+    under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`, each scaled by
+    what the counter reads for the run's own 1 GiB device copies.  Returns a
+    dict, or None when rocprofv3 is missing or a pass fails."""
+    sys.path.insert(0, PROF_DIR)
+    import pmc_summary
+    sys.path.pop(0)
+    exe = shutil.which("rocprofv3")
+    if exe is None:
         return None
-    with open(TRAFFIC_SUMMARY) as f:
-        for line in f:
-            if line.startswith("k_advance_q_hbm_bytes_per_dispatch:"):
-                return float(line.split(":")[1])
-    return None
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="wr_pmc_")
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--",
+                   sys.executable, os.path.join(PROF_DIR, "traffic_run.py"), "--grid", str(grid),
+                   "--max-outer", str(max_outer), "--threads", str(shape[0]), "--per-cu",
+                   str(shape[1]), "--solo", str(shape[2])]
+            if len(shape) > 3:
+                cmd += ["--coop", str(shape[3])]
+            env = dict(os.environ, TMPDIR="/tmp")
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd="/tmp",
+                               env=env)
+            csvs = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs
+                    if f.endswith("counter_collection.csv")]
+            if r.returncode != 0 and not csvs:
+                print(f"# traffic: rocprofv3 {counter} failed ({r.returncode}): "
+                      f"{r.stderr[-300:]}", file=sys.stderr, flush=True)
+                return None
+            summ = {}
+            for c in csvs:
+                summ.update(pmc_summary.summarize(c))
+            copy = [v[counter] for k, v in summ.items() if "copyBuffer" in k and counter in v]
+            kern = [v[counter] for k, v in summ.items() if kernel_name in k and counter in v]
+            if not copy or not kern:
+                print(f"# traffic: no {counter} rows for the copy or {kernel_name}",
+                      file=sys.stderr, flush=True)
+                return None
+            # copy: 1 GiB read and 1 GiB written per dispatch
+            scale = (1 << 20) / copy[0][1]
+            out[counter] = {"dispatches": kern[0][0], "kib_per_dispatch_raw": kern[0][1],
+                            "copy_kib_raw": copy[0][1], "scale": scale,
+                            "bytes_per_dispatch": kern[0][1] * scale * 1024.0}
+    except subprocess.TimeoutExpired:
+        print("# traffic: rocprofv3 pass timed out", file=sys.stderr, flush=True)
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    total = out["FETCH_SIZE"]["bytes_per_dispatch"] + out["WRITE_SIZE"]["bytes_per_dispatch"]
+    return {"bytes_per_dispatch": total, "passes": out}
 
 
 def network_op_counts():
@@ -84,27 +131,71 @@ def run_flops(stats_before, stats_after):
             + attempts * w["attempt"]), d
 
 
-def cpu_baseline(budget_s=20.0):
-    """The reference CPU binary (oracle/_ref/reproducer_cpu, single thread) on
-    a bounded sample: --grid 1 runs (438 gridwide-identical outer steps each)
-    repeated for about budget_s seconds."""
-    exe = os.path.join(REPO, "oracle", "_ref", "reproducer_cpu")
-    if not os.path.exists(exe):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(init, dts, budget_s=12.0):
+    """The reference's own compiled CPU code (oracle/_ref/libwr_ref.so, built
+    from reproducer.cpp) on a bounded, strided sample of the same workload:
+    cells of the 64^3 run replayed through the GPU run's dt_grid of every
+    outer step (the reference's prepare + advance kernel bodies in its host
+    loop's order), about budget_s seconds on 1 thread and again on all the
+    host threads the box gives this process (OpenMP over cells)."""
+    import ctypes
+    so = os.path.join(REPO, "oracle", "_ref", "libwr_ref.so")
+    if not os.path.exists(so):
         return None
-    runs, advances, t0 = 0, 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        out = subprocess.run([exe, "--grid", "1", "--no-compare-final-state"], capture_output=True,
-                             text=True, cwd="/tmp", timeout=120)
-        if out.returncode != 0:
-            return None
-        steps = [int(l.split(":")[1]) for l in out.stdout.splitlines()
-                 if l.startswith("completed global collapse steps")]
-        advances += steps[0]
-        runs += 1
-    el = time.perf_counter() - t0
-    return {"value": advances / el, "unit": "cell-advances/s", "cores": 1, "kind": "reference",
-            "sample": f"{runs} runs of the reference CPU build at --grid 1 (1 cell x 438 outer "
-                      f"steps, legacy-local = gridwide for one cell), {el:.1f} s on 1 core"}
+    lib = ctypes.CDLL(so)
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    fn = lib.ref_replay_cells
+    fn.argtypes = [vp, vp, i32, vp, i32, i32, vp, i32]
+    fn.restype = ctypes.c_longlong
+    n = len(init)
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    order = (np.arange(n, dtype=np.int64) * 104729) % n  # prime stride over the grid
+    nthreads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16),
+                          os.cpu_count() or 1, 16))
+
+    def measure(threads, chunk):
+        k, adv, att, t0 = 0, 0, 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget_s and k < n:
+            idx = np.ascontiguousarray(order[k:k + chunk], dtype=np.int32)
+            cells = init[idx].copy()
+            status = np.zeros(len(idx), dtype=np.int32)
+            att += fn(cells.ctypes.data, idx.ctypes.data, len(idx), dts.ctypes.data, len(dts), 1,
+                      status.ctypes.data, threads)
+            if not np.all(status == 1):
+                return None
+            adv += int(cells["completed_steps"].sum())
+            k += len(idx)
+        el = time.perf_counter() - t0
+        return {"cells": k, "cell_advances": adv, "attempts": att, "seconds": el,
+                "value": adv / el, "attempts_per_s": att / el}
+
+    one = measure(1, 8)
+    allc = measure(nthreads, 8 * nthreads)
+    if one is None or allc is None:
+        return None
+    return {"value": allc["value"], "unit": "cell-advances/s", "cores": nthreads,
+            "kind": "reference",
+            "attempts_per_s": allc["attempts_per_s"],
+            "per_core": {"value": one["value"], "attempts_per_s": one["attempts_per_s"],
+                         "cores": 1, "cells": one["cells"], "seconds": one["seconds"]},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{allc['cells']} cells of the {round(n ** (1 / 3))}^3 run (prime-stride "
+                      f"sample) each replayed through all {len(dts)} outer steps of the GPU run's "
+                      f"dt_grid with the reference's compiled prepare/advance/ROS2S code: "
+                      f"{allc['cell_advances']} cell-advances, {allc['attempts']} ROS2S attempts in "
+                      f"{allc['seconds']:.1f} s on {nthreads} threads; 1 thread: {one['cells']} "
+                      f"cells in {one['seconds']:.1f} s"}
 
 
 def main():
@@ -115,7 +206,10 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--max-outer", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of each of the two CPU-baseline measurements (1 / all threads)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic")
     ap.add_argument("--advance-kernel", type=int, default=1,
                     help="1: queue-fed persistent kernel (default); 0: one thread per cell")
     ap.add_argument("--recipe", default="reference",
@@ -130,6 +224,8 @@ def main():
                     help="skip the Jacobian-reuse study runs")
     ap.add_argument("--jac-ages", default="2,4",
                     help="accepted steps per Jacobian of the Jacobian-reuse study runs")
+    ap.add_argument("--no-lognormal", action="store_true",
+                    help="skip the 128^3 log-normal recipe measurement")
     ap.add_argument("--no-legacy", action="store_true",
                     help="skip the legacy-local policy run")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -172,6 +268,8 @@ def main():
                  "steps_per_candidate": min(args.tune_steps, args.max_outer), "candidates": table}
         print(f"# autotune: {chosen} of {len(table)} shapes", file=sys.stderr, flush=True)
 
+    ctx_shape = ctx.queue_launch()[:3] if args.advance_kernel == 1 else None
+
     def one_run():
         ctx.upload(init)  # inputs resident in HBM before the clock starts
         t0 = time.perf_counter()
@@ -209,7 +307,9 @@ def main():
     achieved = per_launch_flops / mean_launch_s / 1e12 if adv_launches else None
     value = total_adv / total_s
     line = {
-        "metric": f"cell-advances/s, {args.grid}^3 gridwide-independent ROS2S collapse (plain FP64)",
+        "metric": f"cell-advances/s, {args.grid}^3 gridwide-independent ROS2S collapse (FP64; "
+                  f"tuned queue-fed kernel, final states bit-identical to the plain "
+                  f"thread-per-cell kernel)",
         "value": value,
         "unit": "cell-advances/s",
         "n_gpus": 1,
@@ -228,8 +328,8 @@ def main():
                    "cell_advances_per_run": infos[-1]["cell_advances"],
                    "run_seconds": total_s / args.steps,
                    "h200_reference_run_seconds": 17.2347},
-        "roofline": {"bound": "mfma", "compute_pipe": "VALU FP64 (MI355X FP64 peak 78.6 TFLOP/s, "
-                     "vector = matrix)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "valu_fp64", "compute_pipe": "VALU FP64 (vector FP64 instructions "
+                     "only: no MFMA; MI355X FP64 peak 78.6 TFLOP/s, vector = matrix)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
                      "traffic": None,
                      "kernel": "k_advance_q" if args.advance_kernel == 1 else "k_advance",
@@ -239,11 +339,16 @@ def main():
     }
     if tuned:
         line["autotune"] = tuned
-    kname = line["roofline"]["kernel"]
-    traffic = measured_traffic(kname, args.grid) if args.recipe == "reference" else None
-    line["roofline"]["traffic"] = traffic
-    if traffic and adv_launches:
-        line["roofline"]["traffic_gbs"] = traffic / mean_launch_s / 1e9
+    dts = ctx.dt_history()
+    # the reference-equivalent region (reproducer.cpp:8626-8650): device
+    # allocation, host-to-device copy, the run, device-to-host copy, free
+    host_cells = init.copy()
+    t0 = time.perf_counter()
+    r_eq, _ = wr.run_cells(host_cells, perturb=True)
+    line["config"]["reference_equivalent_region_seconds"] = time.perf_counter() - t0
+    line["config"]["reference_equivalent_region_note"] = (
+        "wr_run_cells on host cells: context create (hipMalloc), upload, run, download, destroy; "
+        "status %d" % r_eq)
     if not args.no_plain and args.advance_kernel == 1:
         # the same run with the thread-per-cell kernel: time and bit identity
         ctx.set_advance_kernel(0)
@@ -272,10 +377,25 @@ def main():
                                                    ages)
     if not args.no_legacy and args.advance_kernel == 1:
         line["legacy_local"] = legacy_measure(ctx, init, args.max_outer)
+    if not args.no_lognormal and args.advance_kernel == 1 and args.grid == 64:
+        line["lognormal128"] = lognormal128_measure(wr, ctx_shape)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_measure(wr)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(init, dts, args.cpu_budget)
+    kname = line["roofline"]["kernel"]
+    if not args.no_traffic and args.advance_kernel == 1 and args.recipe == "reference":
+        ctx.close()
+        shape = ctx_shape if ctx_shape else (256, 0, 64)
+        tr = live_traffic(args.grid, args.max_outer, shape, kname)
+        if tr:
+            line["roofline"]["traffic"] = tr["bytes_per_dispatch"]
+            line["roofline"]["traffic_passes"] = tr["passes"]
+            if adv_launches:
+                line["roofline"]["traffic_gbs"] = tr["bytes_per_dispatch"] / mean_launch_s / 1e9
+            line["roofline"]["algorithmic_state_bytes_per_dispatch"] = n * 424
+        print(f"# traffic: {tr['bytes_per_dispatch'] if tr else None} B per dispatch",
+              file=sys.stderr, flush=True)
     print(json.dumps(line))
 
 
@@ -307,6 +427,9 @@ def mixed_measure(ctx, one_run, fp64_cells, fp64_value, n, wr):
     out["T_rel_err_max_median"] = list(rel_errors(got["current"]["T"], fp64_cells["current"]["T"]))
     for f in ("accepted_steps", "rejected_steps"):
         out[f"delta_{f}"] = int(got["stats"][f].sum()) - int(fp64_cells["stats"][f].sum())
+    worst = max([v[0] for k, v in per.items() if "D" not in k] + [out["T_rel_err_max_median"][0]])
+    out["budget"] = "<= 1e-6 relative error in every non-deuterium species and T, >= 2x FP64 speed"
+    out["meets_budget"] = bool(worst <= 1.0e-6 and out["speed_vs_fp64"] >= 2.0)
     print(f"# mixed run: {el:.3f} s, fallbacks {fb}", file=sys.stderr, flush=True)
     return out
 
@@ -352,6 +475,60 @@ def jac_reuse_measure(ctx, one_run, fp64_cells, fp64_value, n, grid, wr, ages):
         shutil.rmtree(tmp, ignore_errors=True)
     return {"rule": "J at a burn's first attempt and after age_max accepted steps; "
                     "refreshed when an attempt with a lagged J is rejected", "runs": runs}
+
+
+def lognormal128_measure(wr, shape, steps=145, compare_steps=20):
+    """The 128^3 log-normal recipe (seed 1, n_H = exp(z) cm^-3, T log-uniform in
+    [100, 8000] K: substep counts per cell differ 10-100x): the queue-fed
+    kernel for `steps` outer steps (about the 1e6 density growth at ~10 % per
+    step), and both kernels on identical inputs for `compare_steps` steps
+    (time and bit identity of the final states)."""
+    n = 128 ** 3
+    seed, sigma, t0, t1 = wr.RECIPES["lognormal128"]
+    init = wr.make_recipe_states(n, seed, sigma, t0, t1)
+    out = {"workload": f"128^3 = {n} cells, lognormal128 recipe (seed {seed}, sigma {sigma}, "
+                       f"T {t0:g}-{t1:g} K), perturbations on"}
+    with wr.Context(n) as ctx:
+        if shape:
+            ctx.set_queue_launch(*shape[:3])
+        ctx.upload(init)
+        ctx.run(perturb=True, max_steps=1)  # prime
+        ctx.upload(init)
+        ta = time.perf_counter()
+        info = ctx.run(perturb=True, max_steps=steps)
+        el = time.perf_counter() - ta
+        cells = np.zeros(n, dtype=wr.CELL_DTYPE)
+        ctx.download(cells)
+        growth = float(cells["density_driver"].max() / init["density_driver"].max())
+        att = cells["stats"]["internal_steps"]
+        out["queue_fed"] = {"outer_steps": info["completed_global_steps"], "run_seconds": el,
+                            "cell_advances": info["cell_advances"],
+                            "value": info["cell_advances"] / el,
+                            "densest_cell_growth": growth,
+                            "attempts_per_cell_min_median_max": [int(att.min()), int(np.median(att)),
+                                                                 int(att.max())]}
+        res = {}
+        for kern in (1, 0):
+            ctx.set_advance_kernel(kern)
+            ctx.upload(init)
+            ta = time.perf_counter()
+            inf = ctx.run(perturb=True, max_steps=compare_steps)
+            e = time.perf_counter() - ta
+            c = np.zeros(n, dtype=wr.CELL_DTYPE)
+            ctx.download(c)
+            res[kern] = (inf, e, c)
+        ctx.set_advance_kernel(1)
+    mask = np.ones(216, bool)
+    mask[156:160] = False
+    same = bool(np.array_equal(res[1][2].view(np.uint8).reshape(n, 216)[:, mask],
+                               res[0][2].view(np.uint8).reshape(n, 216)[:, mask]))
+    out["identical_inputs"] = {"outer_steps": compare_steps,
+                               "queue_fed_seconds": res[1][1], "plain_seconds": res[0][1],
+                               "speedup": res[0][1] / res[1][1], "bit_identical": same}
+    print(f"# lognormal128: {steps} steps {el:.3f} s (growth {growth:.3g}); {compare_steps} steps "
+          f"queue {res[1][1]:.3f} s vs plain {res[0][1]:.3f} s, identical {same}",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def legacy_measure(ctx, init, max_outer):

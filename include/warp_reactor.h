@@ -137,12 +137,20 @@ int wr_download_cells(wr_context *ctx, wr_collapse_state *cells, int num_cells);
 
 /* Run the outer loop on the resident cells from global step
  * `completed_global_steps` and grid time `grid_time` (normally 0, 0) for at
- * most `max_steps` steps (<= 0: 1000).  The outer-step index used by the
- * perturbation schedule is the completed-step count, as in the reference
- * loop (reproducer.cpp:7859, 7957).  Host-side control is device resident:
- * the host only polls a completion flag every few steps. */
+ * most `max_steps` steps (<= 0: 1000).  Resume semantics: the outer-loop
+ * index that the perturbation schedule reads (every 20th step,
+ * reproducer.cpp:7564) starts at the completed count, so a run split into
+ * pieces perturbs at the same steps as one run.  (run_cells_cuda itself starts
+ * its loop index at 0 whatever count the caller passes, 7859; wr_run_cells
+ * and wr_run_from(..., 0) do the same.)  Host-side control is device
+ * resident: the host only polls a completion flag every few steps. */
 int wr_run(wr_context *ctx, int perturb, int max_steps, int completed_global_steps,
            double grid_time);
+/* wr_run with an explicit loop index for the first step: first_step = 0 is
+ * run_cells_cuda's loop (reproducer.cpp:7859-7959) for any completed count;
+ * first_step = completed_global_steps is wr_run. */
+int wr_run_from(wr_context *ctx, int perturb, int max_steps, int completed_global_steps,
+                double grid_time, int first_step);
 int wr_run_info_get(const wr_context *ctx, wr_run_info *out);
 /* dt_grid of each outer step of the last wr_run (the host loop's dt_grid,
  * reproducer.cpp:7888-7897), in step order: copies min(count, max_entries)
@@ -267,6 +275,15 @@ int wr_adjoint_replay(wr_context *ctx, const double *s0, const double *multiplie
 /* gN = dL/d(final state) -> g0 = dL/d(first state), mbar [R][n] = dL/dM_r per
  * cell, at the recorded state and multipliers. */
 int wr_adjoint_backward(wr_context *ctx, const double *gN, double *g0, double *mbar);
+/* Recordings count per context: *generation = the number of successful
+ * wr_adjoint_record calls so far (0: none).  wr_adjoint_backward_gen is
+ * wr_adjoint_backward for the recording with that number, and returns
+ * WR_ERR_ARGUMENT when the context's tape is from another one (a later
+ * forward on the same context overwrote it).  No counterpart in the
+ * reference. */
+int wr_adjoint_generation(const wr_context *ctx, unsigned long long *generation);
+int wr_adjoint_backward_gen(wr_context *ctx, unsigned long long generation, const double *gN,
+                            double *g0, double *mbar);
 /* The recorded tape: hoff [nsteps+1][n] (first tape index of each step),
  * htape [hcap][n], dts [nsteps], ckpt [nsteps][18][n]; any may be NULL. */
 int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double *dts,

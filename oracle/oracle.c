@@ -833,6 +833,30 @@ double or_prepare_cell(or_cell *s, int cell, int completed, double grid_time, in
     return dt;
 }
 
+/* The prepare phase of one outer step over a whole grid (the kernel body on
+ * every cell, reproducer.cpp:7706-7733) and the host loop's minimum of the
+ * candidates (7888-7893, `candidate < dt_grid`, so the lowest cell index
+ * holding the minimum is the one found first).  *cell_out: that cell, or -1
+ * when no cell offers a candidate (*dt_out = DBL_MAX); *failure_out: the
+ * lowest failing cell's code (OR_SUCCESS if none). */
+void or_prepare_grid(or_cell *cells, int n, int completed, double grid_time, int step, int perturb,
+                     double *dt_out, int *cell_out, int *failure_out) {
+    double best = DBL_MAX;
+    int best_cell = -1, failure = OR_SUCCESS;
+    for (int c = 0; c < n; ++c) {
+        int f = OR_SUCCESS;
+        const double cand = or_prepare_cell(&cells[c], c, completed, grid_time, step, perturb, &f);
+        if (f != OR_SUCCESS && failure == OR_SUCCESS) failure = f;
+        if (cand < best) {
+            best = cand;
+            best_cell = c;
+        }
+    }
+    *dt_out = best;
+    *cell_out = best_cell;
+    *failure_out = failure;
+}
+
 /* advance_collapse_gridwide_kernel body, reproducer.cpp:7745-7784 */
 int or_advance_cell(or_cell *s, int completed, double next_grid_time, double dt_grid,
                     int *failure) {

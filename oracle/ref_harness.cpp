@@ -17,6 +17,8 @@
 // arithmetic.
 #include "reproducer.cpp"
 
+#include <omp.h>
+
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -320,6 +322,38 @@ int ref_run_gridwide(void* cells_v, int num_cells, int perturb, int max_steps,
     }
     *completed_global_steps = completed;
     return result;
+}
+
+// Cells of a gridwide run replayed one by one through the run's dt_grid of
+// every outer step: with the grid-wide minimum given, each cell's trajectory
+// is the reference's prepare + advance kernel bodies (7706-7784) in the host
+// loop's order (7859-7959), independent of the other cells.  index[k] is cell
+// k's grid index; status[k] its first failure code (1 = success).  Runs on
+// `threads` OpenMP threads (<= 0: the OpenMP default); returns the ROS2S
+// attempts made (sum of internal_steps added).
+long long ref_replay_cells(void* cells_v, const int* index, int m, const double* dt_hist, int nsteps,
+                           int perturb, int* status, int threads) {
+    auto* cells = static_cast<CollapseState*>(cells_v);
+    const int success = static_cast<int>(IntegratorResult::SUCCESS);
+    long long attempts = 0;
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) reduction(+ : attempts)
+    for (int k = 0; k < m; ++k) {
+        CollapseState& s = cells[k];
+        const unsigned long long before = s.stats.internal_steps;
+        Real t = 0.0;
+        int f = success;
+        for (int step = 0; step < nsteps && f == success; ++step) {
+            prepare_cell(s, index[k], step, t, step, perturb != 0, &f);
+            if (f != success) break;
+            const Real next = t + dt_hist[step];
+            advance_cell(s, step, next, dt_hist[step], &f);
+            t = next;
+        }
+        status[k] = f;
+        attempts += static_cast<long long>(s.stats.internal_steps - before);
+    }
+    return attempts;
 }
 
 // The reference's CPU driver loop (legacy-local policy, reproducer.cpp:8630-8648).

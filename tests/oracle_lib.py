@@ -54,6 +54,8 @@ class _Cpu:
                         "mixed_lu": (_I, [_VP, _D, _VP, _VP, _VP]),
                         "make_recipe_state": (None, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
                         "libm_exp": (_D, [_D]), "libm_log": (_D, [_D]), "libm_cbrt": (_D, [_D]),
+                        "prepare_grid": (None, [_VP, _I, _I, _D, _I, _I, ctypes.POINTER(_D),
+                                                ctypes.POINTER(_I), ctypes.POINTER(_I)]),
                         "replay_cells": (None, [_VP, _VP, _I, _VP, _I, _I, _I, _D, _VP])})
         for name, (res, args) in sig.items():
             fn = getattr(self.lib, p + name)
@@ -198,8 +200,19 @@ class _Cpu:
         self._mixed_fallbacks(_p(out[:1]), _p(out[1:]))
         return int(out[0]), int(out[1])
 
-    def run_gridwide(self, cells, perturb=True, max_steps=0):
-        done = _I(0)
+    def prepare_grid(self, cells, completed, grid_time, step, perturb=True):
+        """prepare of one outer step on every cell (in place) and the host
+        loop's minimum: (dt_grid, lowest cell holding it or -1, failure code)."""
+        dt, cell, f = _D(), _I(), _I()
+        self._prepare_grid(_p(cells), len(cells), completed, grid_time, step, int(perturb),
+                           ctypes.byref(dt), ctypes.byref(cell), ctypes.byref(f))
+        cells["reserved_"] = 0
+        return dt.value, cell.value, f.value
+
+    def run_gridwide(self, cells, perturb=True, max_steps=0, completed0=0):
+        """run_cells_cuda's host loop; the loop index starts at 0 and the
+        completed count at completed0, as in the reference (7791-7859)."""
+        done = _I(completed0)
         r = self._run_gridwide(_p(cells), len(cells), int(perturb), max_steps, ctypes.byref(done))
         cells["reserved_"] = 0
         return r, done.value

@@ -148,6 +148,39 @@ def test_autograd_function_gradients(wr, adj):
     assert np.array_equal(_bits(omb.sum(axis=1)), _bits(gth))
 
 
+def test_autograd_backward_refuses_a_replaced_tape(wr, adj):
+    """Two forwards on one context, then the first one's backward: the
+    context's tape belongs to the second recording, so the first backward
+    must fail instead of differentiating the wrong run; the second forward's
+    backward still works and matches a fresh single-forward run bit for bit."""
+    import importlib
+    import torch
+    ag = importlib.import_module("warp-reactor_amd.autograd")
+    n = 3 ** 3
+    cells, _ = _cells(wr, adj, n)
+    base = ag.state_of(cells)
+    R = wr.adjoint_ntheta()
+    with wr.Context(n) as ctx:
+        assert ctx.adjoint_generation() == 0
+        s0a = torch.tensor(base, dtype=torch.float64, requires_grad=True)
+        tha = torch.zeros(R, dtype=torch.float64, requires_grad=True)
+        la = ag.h2_log_t_loss(ag.collapse(tha, s0a, ctx, nsteps=5))
+        s0b = torch.tensor(base, dtype=torch.float64, requires_grad=True)
+        thb = torch.full((R,), 0.01, dtype=torch.float64, requires_grad=True)
+        lb = ag.h2_log_t_loss(ag.collapse(thb, s0b, ctx, nsteps=5))
+        assert ctx.adjoint_generation() == 2
+        with pytest.raises(wr.WarpReactorError):
+            la.backward()
+        lb.backward()
+        got = thb.grad.numpy().copy(), s0b.grad.numpy().copy()
+    with wr.Context(n) as ctx:
+        s0c = torch.tensor(base, dtype=torch.float64, requires_grad=True)
+        thc = torch.full((R,), 0.01, dtype=torch.float64, requires_grad=True)
+        ag.h2_log_t_loss(ag.collapse(thc, s0c, ctx, nsteps=5)).backward()
+    assert np.array_equal(_bits(got[0]), _bits(thc.grad.numpy()))
+    assert np.array_equal(_bits(got[1]), _bits(s0c.grad.numpy()))
+
+
 def test_record_errors_are_reported(wr):
     """A tape too small for the run's accepted steps is WR_ERR_CAPACITY; a
     cell that stops integrating (its density driver past the collapse stop,

@@ -62,7 +62,30 @@ def test_run64_sample_bit_exact_against_oracle(run64, oracle_wr, wr):
     assert len(bad) == 0, f"{len(bad)} of {len(idx)} sampled cells differ, first {idx[bad[:5]]}"
 
 
+def _rel(a, b, floor=1.0e-20):
+    """|a - b| / max(|b|, floor), 0 where both are below the floor."""
+    a, b = np.asarray(a), np.asarray(b)
+    r = np.abs(a - b) / np.maximum(np.abs(b), floor)
+    r[(np.abs(a) < floor) & (np.abs(b) < floor)] = 0.0
+    return r
+
+
+def _counters(c):
+    return np.stack([c["stats"][k] for k in c["stats"].dtype.names], axis=1)
+
+
 def test_run64_sample_within_reference_comparator(run64, oracle, wr):
+    """Against the reference arithmetic (the oracle with the host libm) after
+    the whole 1000-step run, sampled cells replayed through the run's dt
+    sequence: the reference comparator's tolerances (reproducer.cpp:8416-8549)
+    hold; every controlled species and T is within 1e-9 relative (abundance
+    floor 1e-20) and all seven ROS2S counters are identical on at least 99 %
+    of the sampled cells.  The two libms differ by one ulp on a few per cent of
+    the network's exp/log arguments, which the adaptive controller can grow
+    (on the cells that set dt_grid, by up to ~1e-5 over 1000 steps); the
+    deuterium species, far below the integrator's atol of 1e-4, are not
+    controlled at all -- which is why the reference's comparator does not gate
+    them -- and are reported."""
     init, cells, info, dts = run64
     idx = _sample(len(cells), 128)
     replay = init[idx].copy()
@@ -76,5 +99,71 @@ def test_run64_sample_within_reference_comparator(run64, oracle, wr):
     assert ok.all()
     assert np.all(np.abs(a["current"]["T"] - r["current"]["T"]) <= 1e-4 + 2e-4 * np.abs(r["current"]["T"]))
     assert np.all(np.abs(a["current"]["rho"] - r["current"]["rho"]) <= 1e-4 + 1e-4 * np.abs(r["current"]["rho"]))
-    rel = np.abs(xa - xr) / np.maximum(np.abs(xr), 1e-4)
-    print("max non-D species rel error vs host-libm arithmetic:", np.delete(rel, DEUTERIUM, axis=1).max())
+    rel = _rel(xa, xr)
+    same = np.all(_counters(a) == _counters(r), axis=1)
+    nond = np.delete(rel, DEUTERIUM, axis=1)
+    relT = _rel(a["current"]["T"], r["current"]["T"])
+    print(f"1000 steps vs host-libm arithmetic, {len(idx)} cells: counters identical on "
+          f"{same.mean():.3f}; max rel err (floor 1e-20) non-D {nond.max():.2e}, "
+          f"D {rel[:, DEUTERIUM].max():.2e}, T {relT.max():.2e}")
+    assert same.mean() >= 0.99
+    assert nond.max() <= 1.0e-9 and relT.max() <= 1.0e-9
+
+
+def test_run64_100_steps_within_1e9_of_reference_arithmetic(wr, oracle):
+    """The first 100 outer steps of the default 64^3 run against the reference
+    arithmetic (oracle with the host libm), sampled cells replayed through the
+    GPU run's dt sequence: every controlled species and T within 1e-9 relative
+    (abundance floor 1e-20) and all seven ROS2S counters identical on every
+    sampled cell.  The deuterium species (reproducer.cpp:8462-8473: reported,
+    not gated) are reported."""
+    n, steps = 64 ** 3, 100
+    init = wr.make_collapse_states(n)
+    with wr.Context(n) as ctx:
+        ctx.upload(init)
+        info = ctx.run(perturb=True, max_steps=steps)
+        cells = np.zeros(n, dtype=wr.CELL_DTYPE)
+        ctx.download(cells)
+        dts = ctx.dt_history()
+    assert info["status"] == 1 and len(dts) == steps
+    idx = _sample(n, 512)
+    replay = init[idx].copy()
+    assert np.all(oracle.replay_cells(replay, idx, dts, perturb=True) == 1)
+    a = cells[idx]
+    assert np.array_equal(a["completed_steps"], replay["completed_steps"])
+    assert np.array_equal(_counters(a), _counters(replay))
+    rel = _rel(a["current"]["xn"], replay["current"]["xn"])
+    relT = _rel(a["current"]["T"], replay["current"]["T"])
+    nond = np.delete(rel, DEUTERIUM, axis=1)
+    print(f"100 steps vs host-libm arithmetic, {len(idx)} cells: non-D max {nond.max():.2e}, "
+          f"T max {relT.max():.2e}, D max {rel[:, DEUTERIUM].max():.2e}")
+    assert nond.max() <= 1.0e-9
+    assert relT.max() <= 1.0e-9
+
+
+def test_run64_against_reference_full_grid_fixture(run64, wr):
+    """Against the reference's own compiled arithmetic run over the whole grid
+    (tests/golden/grid64_reference.npz: reproducer.cpp's kernels through its
+    gridwide host loop on the CPU, 64^3 cells, 1000 steps): the sampled
+    records pass the reference comparator's tolerances (8416-8549; deuterium
+    species reported, not gated), with the same completed-step counts."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "grid64_reference.npz")
+    fx = np.load(path)
+    init, cells, info, dts = run64
+    assert info["completed_global_steps"] == int(fx["completed_global_steps"])
+    idx = fx["index"]
+    ref = np.frombuffer(fx["records"].tobytes(), dtype=wr.PACKED_DTYPE)
+    assert np.array_equal(ref["cell"], idx)
+    got = wr.pack_final_states(cells, 64)[idx]
+    assert np.array_equal(got["completed_steps"], ref["completed_steps"])
+    xa, xr = got["xn"], ref["xn"]
+    ok = np.abs(xa - xr) <= 1e-4 + SPECIES_RTOL * np.abs(xr)
+    ok[:, DEUTERIUM] = True
+    assert ok.all()
+    for f, rt in (("T", 2e-4), ("e", 2e-4), ("rho", 1e-4)):
+        assert np.all(np.abs(got[f] - ref[f]) <= 1e-4 + rt * np.abs(ref[f])), f
+    rel = _rel(xa, xr)
+    print(f"64^3 x {info['completed_global_steps']} vs reference full-grid run ({len(idx)} sampled "
+          f"records, sha256 {str(fx['sha256'])[:16]}): non-D max {np.delete(rel, DEUTERIUM, axis=1).max():.2e}, "
+          f"T max {_rel(got['T'], ref['T']).max():.2e}, time equal {np.mean(got['time'] == ref['time']):.3f}")

@@ -59,12 +59,16 @@ class CollapseAdjoint(torch.autograd.Function):
         context.adjoint_record(nsteps, M, hcap)
         context.download(cells)
         fctx.context = context
+        fctx.generation = context.adjoint_generation()  # the tape this forward recorded
         fctx.M = M
         return torch.from_numpy(state_of(cells)).to(device=s0.device)
 
     @staticmethod
     def backward(fctx, gN):
-        g0, mbar = fctx.context.adjoint_backward(gN.detach().to(torch.float64).cpu().numpy())
+        # fails (WarpReactorError) if a later forward on the same context
+        # replaced this forward's tape
+        g0, mbar = fctx.context.adjoint_backward(gN.detach().to(torch.float64).cpu().numpy(),
+                                                 generation=fctx.generation)
         gtheta = mbar.sum(axis=1) * fctx.M  # dM/dtheta = M
         dev = gN.device
         return (torch.from_numpy(gtheta).to(device=dev), torch.from_numpy(g0).to(device=dev),

@@ -30,6 +30,8 @@
 
 #include <string.h>
 
+#include <string>
+
 #include <vector>
 
 namespace wr {
@@ -514,6 +516,9 @@ struct AdjointState {
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
     bool recorded = false;
+    // number of successful recordings on this context: a backward pass names
+    // the recording it differentiates (wr_adjoint_backward_gen)
+    unsigned long long generation = 0;
     double* ckpt = nullptr;
     int* hoff = nullptr;
     double* htape = nullptr;
@@ -701,7 +706,26 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
                      hipMemcpyDeviceToHost),
            "hipMemcpy(adjoint tape index)");
     a->recorded = true;
+    a->generation += 1;
     return kSuccess;
+}
+
+int wr_adjoint_generation(const wr_context* ctx, unsigned long long* generation) {
+    if (!ctx || !generation) return fail(WR_ERR_ARGUMENT, "wr_adjoint_generation: bad argument");
+    *generation = (ctx->adj && ctx->adj->recorded) ? ctx->adj->generation : 0ull;
+    return WR_SUCCESS;
+}
+
+int wr_adjoint_backward_gen(wr_context* ctx, unsigned long long generation, const double* gN,
+                            double* g0, double* mbar) {
+    if (!ctx || !ctx->adj || !ctx->adj->recorded)
+        return fail(WR_ERR_ARGUMENT, "wr_adjoint_backward_gen: nothing recorded");
+    if (ctx->adj->generation != generation)
+        return fail(WR_ERR_ARGUMENT,
+                    "wr_adjoint_backward_gen: the context's tape is from another recording "
+                    "(recorded " + std::to_string(ctx->adj->generation) + ", asked for " +
+                        std::to_string(generation) + ")");
+    return wr_adjoint_backward(ctx, gN, g0, mbar);
 }
 
 int wr_adjoint_tape(const wr_context* ctx, int32_t* hoff, double* htape, double* dts,

@@ -711,15 +711,19 @@ int wr_download_cells(wr_context* ctx, wr_collapse_state* cells, int num_cells) 
     return WR_SUCCESS;
 }
 
-static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time) {
+// first_step: the outer-loop index of the run's first step, which the
+// perturbation schedule reads (reproducer.cpp:7859 starts it at 0 whatever
+// completed count the caller passes); it advances with the completed count.
+static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time,
+                    int first_step) {
     Ctl h{};
     h.grid_time = grid_time;
     h.failure = kSuccess;
     h.completed = completed;
-    h.step = completed;  // the reference's loop index equals the completed count
-    h.step_limit = completed + max_steps;
+    h.step = first_step;
+    h.step_limit = first_step + max_steps;
     h.failed_cell = -1;
-    h.step0 = completed;
+    h.step0 = first_step;
     h.hist_cap = wr_context::kHistCap;
     h.dt_hist = ctx->dt_hist;
     h.done = (max_steps <= 0) ? 1 : 0;
@@ -974,14 +978,22 @@ static int run_legacy(wr_context* ctx, int max_steps) {
 
 int wr_run(wr_context* ctx, int perturb, int max_steps, int completed_global_steps,
            double grid_time) {
+    return wr_run_from(ctx, perturb, max_steps, completed_global_steps, grid_time,
+                       completed_global_steps);
+}
+
+int wr_run_from(wr_context* ctx, int perturb, int max_steps, int completed_global_steps,
+                double grid_time, int first_step) {
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_run: null context");
+    if (first_step < 0) return fail(WR_ERR_ARGUMENT, "wr_run_from: negative first step");
     if (max_steps <= 0) max_steps = 1000;
     WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     if (ctx->policy == WR_POLICY_LEGACY_LOCAL && (ctx->lu_mode != 0 || ctx->jac_age_max > 1))
         return fail(WR_ERR_ARGUMENT,
                     "wr_run: the legacy-local policy runs the reference integrator "
                     "(wr_set_lu_precision 0, wr_set_jacobian_reuse 1)");
-    int st = init_ctl(ctx, perturb, ctx->n > 0 ? max_steps : 0, completed_global_steps, grid_time);
+    int st = init_ctl(ctx, perturb, ctx->n > 0 ? max_steps : 0, completed_global_steps, grid_time,
+                      first_step);
     if (st != WR_SUCCESS) return st;
     ctx->adv_ms = 0.0;
     ctx->adv_launches = 0;
@@ -1142,17 +1154,13 @@ int wr_prepare(wr_context* ctx, int completed_global_steps, double grid_time, in
                double* dt_grid, int* limiting_cell) {
     if (!ctx || !dt_grid) return fail(WR_ERR_ARGUMENT, "wr_prepare: bad argument");
     WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
-    int st = init_ctl(ctx, perturb, step + 1, completed_global_steps, grid_time);
+    int st = init_ctl(ctx, perturb, 1, completed_global_steps, grid_time, step);
     if (st != WR_SUCCESS) return st;
     if (ctx->n == 0) {
         *dt_grid = DBL_MAX;
         if (limiting_cell) *limiting_cell = -1;
         return WR_SUCCESS;
     }
-    // the control block carries the step index used for the perturbation
-    ctx->ctl_host->step = step;
-    WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
-           "hipMemcpy(control to device)");
     if ((st = launch_prepare(ctx)) != WR_SUCCESS) return st;
     if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
     const Ctl& h = *ctx->ctl_host;
@@ -1175,7 +1183,7 @@ int wr_advance(wr_context* ctx, int completed_global_steps, double next_grid_tim
                int* integrated) {
     if (!ctx) return fail(WR_ERR_ARGUMENT, "wr_advance: null context");
     WR_HIP(hipSetDevice(ctx->device), "hipSetDevice");
-    int st = init_ctl(ctx, 0, 1, completed_global_steps, 0.0);
+    int st = init_ctl(ctx, 0, 1, completed_global_steps, 0.0, 0);
     if (st != WR_SUCCESS) return st;
     ctx->ctl_host->dt_grid = dt_grid;
     ctx->ctl_host->next_grid_time = next_grid_time;
@@ -1202,7 +1210,8 @@ int wr_run_cells(wr_collapse_state* cells, int num_cells, int perturb, int* comp
         wr_context_destroy(ctx);
         return WR_BAD_INPUTS;
     }
-    st = wr_run(ctx, perturb, 1000, *completed_global_steps, 0.0);
+    // the reference's loop index starts at 0 (reproducer.cpp:7859)
+    st = wr_run_from(ctx, perturb, 1000, *completed_global_steps, 0.0, 0);
     int result = WR_SUCCESS;
     if (st <= WR_ERR_HIP) {
         fprintf(stderr, "%s\n", g_last_error.c_str());
@@ -1419,7 +1428,7 @@ int wr_eval_integrate(wr_context* ctx, double* y, const double* tout, const doub
 namespace wr {
 namespace detail {
 int ctl_init(wr_context* ctx, int perturb, int max_steps, int completed, double grid_time) {
-    return ::init_ctl(ctx, perturb, max_steps, completed, grid_time);
+    return ::init_ctl(ctx, perturb, max_steps, completed, grid_time, completed);
 }
 int ctl_read(wr_context* ctx) { return ::read_ctl(ctx); }
 int ctl_prepare(wr_context* ctx) { return ::launch_prepare(ctx); }

@@ -74,6 +74,7 @@ _SIGS = {
     "wr_upload_cells": (_I, [_VP, _VP, _I]),
     "wr_download_cells": (_I, [_VP, _VP, _I]),
     "wr_run": (_I, [_VP, _I, _I, _I, _D]),
+    "wr_run_from": (_I, [_VP, _I, _I, _I, _D, _I]),
     "wr_run_info_get": (_I, [_VP, ctypes.POINTER(RunInfo)]),
     "wr_prepare": (_I, [_VP, _I, _D, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
@@ -102,6 +103,8 @@ _SIGS = {
     "wr_adjoint_record": (_I, [_VP, _I, _VP, _I, ctypes.POINTER(_I)]),
     "wr_adjoint_replay": (_I, [_VP, _VP, _VP, _VP]),
     "wr_adjoint_backward": (_I, [_VP, _VP, _VP, _VP]),
+    "wr_adjoint_generation": (_I, [_VP, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "wr_adjoint_backward_gen": (_I, [_VP, ctypes.c_ulonglong, _VP, _VP, _VP]),
     "wr_adjoint_tape": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "wr_adjoint_last_ms": (_I, [_VP, ctypes.POINTER(_D)]),
     "wr_adjoint_set_window": (_I, [_VP, _I]),
@@ -170,12 +173,12 @@ def make_recipe_states(n, seed=0, sigma_ln_nh=0.1, t_min=300.0, t_max=None):
     return cells
 
 
-def run_cells(cells, perturb=True):
+def run_cells(cells, perturb=True, completed_global_steps=0):
     """run_cells_cuda (reproducer.cpp:7791): advances `cells` in place.
 
     Returns (IntegratorResult code, completed_global_steps)."""
     assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
-    completed = ctypes.c_int(0)
+    completed = ctypes.c_int(int(completed_global_steps))
     r = library().wr_run_cells(_ptr(cells), len(cells), int(bool(perturb)), ctypes.byref(completed))
     return r, completed.value
 
@@ -245,9 +248,15 @@ class Context:
         assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
         _check(self._lib.wr_download_cells(self._h, _ptr(cells), len(cells)), "wr_download_cells")
 
-    def run(self, perturb=True, max_steps=1000, completed=0, grid_time=0.0):
-        """Outer loop on resident cells; returns the run info as a dict."""
-        r = self._lib.wr_run(self._h, int(bool(perturb)), max_steps, completed, grid_time)
+    def run(self, perturb=True, max_steps=1000, completed=0, grid_time=0.0, first_step=None):
+        """Outer loop on resident cells; returns the run info as a dict.
+        first_step: loop index of the first step (None: the completed count,
+        wr_run's resume semantics; 0: run_cells_cuda's loop)."""
+        if first_step is None:
+            r = self._lib.wr_run(self._h, int(bool(perturb)), max_steps, completed, grid_time)
+        else:
+            r = self._lib.wr_run_from(self._h, int(bool(perturb)), max_steps, completed,
+                                      grid_time, int(first_step))
         if r <= ERR_HIP:
             _check(r, "wr_run")
         info = RunInfo()
@@ -399,12 +408,25 @@ class Context:
             _check(r, "wr_adjoint_replay")
         return sN
 
-    def adjoint_backward(self, gN):
+    def adjoint_generation(self):
+        """Number of successful recordings on this context (0: none)."""
+        g = ctypes.c_ulonglong()
+        _check(self._lib.wr_adjoint_generation(self._h, ctypes.byref(g)), "wr_adjoint_generation")
+        return g.value
+
+    def adjoint_backward(self, gN, generation=None):
+        """dL/d(first state), dL/dM per cell; `generation` (adjoint_generation()
+        right after the recording) makes the call fail if the context's tape
+        has since been replaced by another recording."""
         n = self._n
         g0 = np.zeros((18, n))
         mb = np.zeros((adjoint_ntheta(), n))
         g = np.ascontiguousarray(gN, dtype=np.float64)
-        r = self._lib.wr_adjoint_backward(self._h, _ptr(g), _ptr(g0), _ptr(mb))
+        if generation is None:
+            r = self._lib.wr_adjoint_backward(self._h, _ptr(g), _ptr(g0), _ptr(mb))
+        else:
+            r = self._lib.wr_adjoint_backward_gen(self._h, int(generation), _ptr(g), _ptr(g0),
+                                                  _ptr(mb))
         if r != SUCCESS:
             _check(r, "wr_adjoint_backward")
         return g0, mb
