@@ -264,11 +264,11 @@ def main():
         autotune = importlib.import_module("warp-reactor_amd.autotune")
         chosen, table = autotune.autotune(ctx, init, steps=min(args.tune_steps, args.max_outer))
         tuned = {"chosen": {"threads": chosen[0], "blocks_per_cu": chosen[1],
-                            "solo_lanes": chosen[2]},
+                            "solo_lanes": chosen[2], "coop_cells": chosen[3]},
                  "steps_per_candidate": min(args.tune_steps, args.max_outer), "candidates": table}
         print(f"# autotune: {chosen} of {len(table)} shapes", file=sys.stderr, flush=True)
 
-    ctx_shape = ctx.queue_launch()[:3] if args.advance_kernel == 1 else None
+    ctx_shape = (ctx.queue_launch()[:3] + (ctx.coop()[0],)) if args.advance_kernel == 1 else None
 
     def one_run():
         ctx.upload(init)  # inputs resident in HBM before the clock starts
@@ -386,7 +386,7 @@ def main():
     kname = line["roofline"]["kernel"]
     if not args.no_traffic and args.advance_kernel == 1 and args.recipe == "reference":
         ctx.close()
-        shape = ctx_shape if ctx_shape else (256, 0, 64)
+        shape = ctx_shape if ctx_shape else (256, 0, 64, 0)
         tr = live_traffic(args.grid, args.max_outer, shape, kname)
         if tr:
             line["roofline"]["traffic"] = tr["bytes_per_dispatch"]
@@ -491,6 +491,8 @@ def lognormal128_measure(wr, shape, steps=145, compare_steps=20):
     with wr.Context(n) as ctx:
         if shape:
             ctx.set_queue_launch(*shape[:3])
+            if len(shape) > 3:
+                ctx.set_coop(shape[3], shape[3])
         ctx.upload(init)
         ctx.run(perturb=True, max_steps=1)  # prime
         ctx.upload(init)

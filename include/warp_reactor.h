@@ -198,6 +198,21 @@ int wr_set_advance_kernel(wr_context *ctx, int kernel);
  * (warp-reactor_amd/autotune.py) picks it by timing.  Defaults 256, 0, 64. */
 int wr_set_queue_launch(wr_context *ctx, int threads_per_block, int blocks_per_cu,
                         int solo_lanes);
+/* Wave-cooperative advance of the most expensive cells (no counterpart in the
+ * reference, whose advance is one thread per cell, reproducer.cpp:7736): in
+ * every outer step the first `cells` queue positions -- the cells that took
+ * the most ROS2S attempts in the previous step -- are advanced by whole
+ * 64-lane waves, one cell per wave, taken by wave 0 of the first `waves`
+ * blocks of the queue-fed kernel; the network runs as cooperative programs
+ * over the wave's LDS, the LU and solves at wave level.  The single-lane
+ * path takes the other cells, and the cooperative waves join it when their
+ * cells are done.  Results are bit-identical to the single-lane path.
+ * Applies to the queue-fed FP64 kernel with the reference's Jacobian rule
+ * and the gridwide policy; cells = 0 or waves = 0 turns it off. */
+int wr_set_coop(wr_context *ctx, int cells, int waves);
+/* The current setting and the LDS bytes a cooperative wave uses. */
+int wr_get_coop(const wr_context *ctx, int *cells, int *waves, int *lds_bytes);
+
 /* The current shape and the grid size last used (0 before the first launch). */
 int wr_get_queue_launch(const wr_context *ctx, int *threads_per_block, int *blocks_per_cu,
                         int *solo_lanes, int *blocks);

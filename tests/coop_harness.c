@@ -42,9 +42,9 @@ static inline double coop_sgn(double x, uint32_t w) {
     }
 #define COOP_TW(tab, row) (tab)[(row) * 64 + ln]
 #define COOP_LD(a) (*(const double *)(coop_smem + (a)))
-#define COOP_LDU(a) (*(const uint32_t *)(coop_smem + ((a) & COOP_ADDR_MASK)))
-#define COOP_ST(a, v) (*(double *)(coop_smem + ((a) & COOP_ADDR_MASK)) = (v))
-#define COOP_STU(a, v) (*(uint32_t *)(coop_smem + ((a) & COOP_ADDR_MASK)) = (v))
+#define COOP_LDU(a) (*(const uint32_t *)(coop_smem + (a)))
+#define COOP_ST(a, v) (*(double *)(coop_smem + (a)) = (v))
+#define COOP_STU(a, v) (*(uint32_t *)(coop_smem + (a)) = (v))
 #include "coop_network.h"
 
 static void coop_load_inputs(double T, const double *X) {

@@ -48,6 +48,8 @@ def test_autotuner_sets_its_choice(wr):
     init = wr.make_collapse_states(n)
     with wr.Context(n) as ctx:
         chosen, table = autotune.autotune(ctx, init, steps=20)
-        assert chosen in [(r["threads"], r["blocks_per_cu"], r["solo_lanes"]) for r in table]
+        assert chosen in [(r["threads"], r["blocks_per_cu"], r["solo_lanes"], r["coop_cells"])
+                          for r in table]
         assert len(table) == len(list(autotune.candidates()))
-        assert ctx.queue_launch()[:3] == chosen
+        assert ctx.queue_launch()[:3] == chosen[:3]
+        assert ctx.coop()[:2] == (chosen[3], chosen[3])

@@ -56,6 +56,9 @@ def generate():
     out = os.path.join(CSRC, "_gen", "chem_network.h")
     counts = os.path.join(CSRC, "_gen", "chem_counts.json")
     run([sys.executable, os.path.join(GEN, "emit.py"), out, counts])
+    # wave-cooperative programs of the same network (wr_coop.hpp)
+    run([sys.executable, os.path.join(GEN, "coop.py"), os.path.join(CSRC, "_gen", "coop_network.h"),
+         os.path.join(CSRC, "_gen", "coop_stats.json")])
     # derivative code: the device subset (the CPU header is the oracle's)
     run([sys.executable, os.path.join(GEN, "ad.py"), os.path.join(CSRC, "_gen", "chem_adjoint_cpu.h"),
          os.path.join(CSRC, "_gen", "chem_adjoint.json"),

@@ -82,6 +82,7 @@ __device__ __forceinline__ void adj_error(int* err, int code) { atomicCAS(err, 0
 // ---- the recording run in the queue-fed kernel: f_theta, checkpoint, tape ----------------
 struct QThetaNet {
     static constexpr int kSlots = AD_FWD_NSLOT;
+    static constexpr bool kCoop = false;  // the recording run advances every cell per lane
     const double* M;
     double* ckpt;  // [nsteps][18][cap]
     int* hoff;     // [nsteps + 1][cap]

@@ -55,6 +55,9 @@ struct Ctl {
     double* dt_hist;    // dt_grid of every outer step of the run (device)
     int queue;          // next queue position to hand out in the queue-fed advance
     int queue_solo;     // next of the positions reserved for the solo wave
+    int queue_coop;     // next of the positions reserved for the cooperative waves
+    int coop_cells;     // queue positions (most expensive cells first) advanced by whole waves
+    int coop_waves;     // blocks whose wave 0 advances those cells
     int integ_max_steps;  // integrator step limit (configure_ros2s: 1e7)
     int lu_sweeps;        // refinement sweeps per stage solve (FP32-factorisation form)
     int solo_lanes;       // queue positions reserved for block 0's first wave (0: none)
@@ -138,13 +141,18 @@ struct wr_context {
     int q_per_cu = 0;
     int q_solo = 64;
     int q_blocks = 0;
+    // wave-cooperative advance of the most expensive cells (wr_set_coop):
+    // queue positions given to whole waves, and how many waves take them
+    int coop_cells = 0;
+    int coop_waves = 0;
     // HIP-event timing of the advance launches of wr_run (when enabled)
     static constexpr int kChunk = 8;
     bool timing = false;
     // one chunk of outer steps captured as a HIP graph (used when not timing)
     bool use_graph = true;
     hipGraphExec_t chunk_exec = nullptr;
-    int graph_n = -1, graph_kernel = -1;  // what the captured launches were built for
+    int graph_n = -1;             // what the captured launches were built for
+    long long graph_kernel = -1;
     hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
     double adv_ms = 0.0;
     int adv_launches = 0;

@@ -18,6 +18,7 @@
 #include "wr_ctx.hpp"
 #include "wr_device.hpp"
 #include "wr_queue.hpp"
+#include "wr_coop.hpp"
 
 #include <float.h>
 #include <stdio.h>
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(Ctl* ctl, const double* par
     ctl->integrated = 0;
     ctl->queue = 0;
     ctl->queue_solo = 0;
+    ctl->queue_coop = 0;
     const int k = ctl->step - ctl->step0;
     if (ctl->dt_hist && k >= 0 && k < ctl->hist_cap) ctl->dt_hist[k] = dt;
 }
@@ -338,6 +340,9 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(DevCells s, Ctl* 
 // The production network: the generated staged J + f and f, no tape.
 struct QPlainNet {
     static constexpr int kSlots = kStageSlots;
+    // the most expensive cells of a step may be advanced by whole waves
+    static constexpr bool kCoop = true;
+    __device__ static void coop(DevCells s, Ctl* ctl, int lane) { coop_run(s, ctl, lane); }
     __device__ __forceinline__ void jac_rhs(const ThreadWs& w) { ws_jac_rhs(w); }
     __device__ __forceinline__ void rhs(const ThreadWs& w) { ws_rhs(w); }
     __device__ __forceinline__ void begin(int, const double (&)[kNumSpec], double, double, double,
@@ -711,6 +716,13 @@ int wr_download_cells(wr_context* ctx, wr_collapse_state* cells, int num_cells) 
     return WR_SUCCESS;
 }
 
+// Whether the wave-cooperative advance applies to subsequent launches: the
+// queue-fed FP64 kernel with the reference's Jacobian rule, gridwide policy.
+static bool coop_active(const wr_context* ctx) {
+    return ctx->coop_cells > 0 && ctx->coop_waves > 0 && ctx->advance_kernel == 1 &&
+           ctx->lu_mode == 0 && ctx->jac_age_max == 1 && ctx->policy == WR_POLICY_GRIDWIDE;
+}
+
 // first_step: the outer-loop index of the run's first step, which the
 // perturbation schedule reads (reproducer.cpp:7859 starts it at 0 whatever
 // completed count the caller passes); it advances with the completed count.
@@ -735,6 +747,10 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.lu_sweeps = ctx->lu_sweeps;
     h.solo_lanes = ctx->q_solo;
     h.jac_age_max = ctx->jac_age_max;
+    if (coop_active(ctx)) {
+        h.coop_cells = ctx->coop_cells;
+        h.coop_waves = ctx->coop_waves;
+    }
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
            "hipMemcpy(control to device)");
@@ -807,8 +823,8 @@ static int launch_advance(wr_context* ctx) {
                                dim3(ctx->q_threads), 0, ctx->stream, ctx->cells, ctx->ctl, 0,
                                QPlainNet{});
         else
-            hipLaunchKernelGGL((k_advance_q<false, QPlainNet>), dim3(ctx->q_blocks), dim3(ctx->q_threads), 0,
-                               ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
+            hipLaunchKernelGGL((k_advance_q<false, QPlainNet>), dim3(ctx->q_blocks), dim3(ctx->q_threads),
+                               0, ctx->stream, ctx->cells, ctx->ctl, 0, QPlainNet{});
     } else {
         hipLaunchKernelGGL(k_advance, dim3(blocks_for(ctx->n, kAdvBlock)), dim3(kAdvBlock), 0,
                            ctx->stream, ctx->cells, ctx->ws, ctx->ctl);
@@ -856,8 +872,9 @@ static int enqueue_steps(wr_context* ctx, int todo, bool timed) {
 // except the cell count and the advance kernel, so the graph is captured
 // again only when those change.  The graph holds kernel nodes only.
 static int chunk_graph(wr_context* ctx) {
-    const int kernel_key = ctx->advance_kernel + 2 * ctx->lu_mode + 4 * (ctx->jac_age_max > 1) +
-                           8 * (ctx->q_threads + 1024 * (ctx->q_per_cu + 64 * ctx->q_solo));
+    const long long kernel_key =
+        ctx->advance_kernel + 2 * ctx->lu_mode + 4 * (ctx->jac_age_max > 1) +
+        8LL * (ctx->q_threads + 1024LL * (ctx->q_per_cu + 64LL * (ctx->q_solo + 128LL * coop_active(ctx))));
     if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == kernel_key)
         return WR_SUCCESS;
     if (ctx->chunk_exec) {
@@ -1096,6 +1113,23 @@ int wr_set_queue_launch(wr_context* ctx, int threads_per_block, int blocks_per_c
     ctx->q_solo = solo_lanes;
     ctx->q_blocks = 0;
     ctx->q_blocks_local = 0;
+    return WR_SUCCESS;
+}
+
+int wr_set_coop(wr_context* ctx, int cells, int waves) {
+    if (!ctx || cells < 0 || waves < 0 || cells > (1 << 20) || waves > 4096)
+        return fail(WR_ERR_ARGUMENT, "wr_set_coop: cells 0..2^20, waves 0..4096");
+    ctx->coop_cells = cells;
+    ctx->coop_waves = waves;
+    ctx->q_blocks = 0;  // occupancy with the cooperative LDS
+    return WR_SUCCESS;
+}
+
+int wr_get_coop(const wr_context* ctx, int* cells, int* waves, int* lds_bytes) {
+    if (!ctx || !cells || !waves) return fail(WR_ERR_ARGUMENT, "wr_get_coop: bad argument");
+    *cells = ctx->coop_cells;
+    *waves = ctx->coop_waves;
+    if (lds_bytes) *lds_bytes = COOP_LDS_BYTES;
     return WR_SUCCESS;
 }
 

@@ -89,6 +89,15 @@ template <bool kMixed, class Net, bool kLocal = false, bool kJacReuse = false>
 __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int zero, Net net) {
     using C = Ros2sCoef;
     if (ctl->done) return;
+    // wave-cooperative cells: the first coop_cells queue positions, taken by
+    // wave 0 of the first coop_waves blocks, one cell per wave (wr_coop.hpp);
+    // those waves then join the per-lane queue below
+    constexpr bool kCoop = Net::kCoop && !kMixed && !kLocal && !kJacReuse;
+    const int coop0 = kCoop ? ctl->coop_cells : 0;
+    if constexpr (kCoop) {
+        if (coop0 > 0 && threadIdx.x < 64 && static_cast<int>(blockIdx.x) < ctl->coop_waves)
+            Net::coop(s, ctl, static_cast<int>(threadIdx.x));
+    }
     const int solo_lanes = ctl->solo_lanes;
     bool solo = blockIdx.x == 0 && solo_lanes > 0;
     if (solo && static_cast<int>(threadIdx.x) >= solo_lanes) return;
@@ -128,8 +137,9 @@ __global__ __launch_bounds__(kQBlock) void k_advance_q(DevCells s, Ctl* ctl, int
                     solo = false;  // reserved positions used up: join the common queue
                     continue;
                 }
+                q += coop0;
             } else {
-                q = solo_lanes + atomicAdd(&ctl->queue, 1);
+                q = coop0 + solo_lanes + atomicAdd(&ctl->queue, 1);
             }
             if (q >= s.n || ctl->failure != kSuccess) break;
             c = s.order[q];

@@ -706,12 +706,20 @@ def emit_function(prog, lay, fname, tabname, outputs_addr, prefetch=8):
         cls = it[1]
         t = lambda i: f"COOP_V(t{k}_{i})"
         M = f"{ADDR_MASK:#x}u"
-        if cls in ("add", "mul", "div", "ge", "gt", "eq", "ne"):
+        # only the second word of an add (subtract flag) and the word of a
+        # sign op carry flags above the address bits
+        if cls == "add":
             decl = [f"COOP_DD(a{k})", f"COOP_DD(b{k})"]
-            body = (f"COOP_V(a{k}) = COOP_LD({t(0)} & {M}); COOP_V(b{k}) = COOP_LD({t(1)} & {M});")
-        elif cls in ("exp", "log", "sqrt", "sgn", "mov"):
+            body = (f"COOP_V(a{k}) = COOP_LD({t(0)}); COOP_V(b{k}) = COOP_LD({t(1)} & {M});")
+        elif cls in ("mul", "div", "ge", "gt", "eq", "ne"):
+            decl = [f"COOP_DD(a{k})", f"COOP_DD(b{k})"]
+            body = (f"COOP_V(a{k}) = COOP_LD({t(0)}); COOP_V(b{k}) = COOP_LD({t(1)});")
+        elif cls == "sgn":
             decl = [f"COOP_DD(a{k})"]
             body = f"COOP_V(a{k}) = COOP_LD({t(0)} & {M});"
+        elif cls in ("exp", "log", "sqrt", "mov"):
+            decl = [f"COOP_DD(a{k})"]
+            body = f"COOP_V(a{k}) = COOP_LD({t(0)});"
         elif cls in ("and", "or"):
             decl = [f"COOP_DU(a{k})", f"COOP_DU(b{k})"]
             body = f"COOP_V(a{k}) = COOP_LDU({t(0)}); COOP_V(b{k}) = COOP_LDU({t(1)});"
@@ -906,6 +914,8 @@ def emit_header(ir_path, out_path, json_path=None):
         " * COOP_DATA, COOP_DU/DD (lane variable declarations), COOP_V (lane variable),",
         " * COOP_L (a lane statement), COOP_TW (table word), COOP_LD/LDU/ST/STU (LDS access",
         " * at a byte address), coop_flip, coop_sgn, CHEM_EXP and CHEM_LOG. */",
+        "#ifndef COOP_LAYOUT_DEFINED",
+        "#define COOP_LAYOUT_DEFINED",
         f"#define COOP_OFF_CONST {lay.const}",
         f"#define COOP_OFF_TRASH {lay.trash}",
         f"#define COOP_OFF_IN {lay.inp}",
@@ -917,8 +927,13 @@ def emit_header(ir_path, out_path, json_path=None):
         f"#define COOP_LDS_BYTES {lay.total}",
         f"#define COOP_NCONST {len(lay.consts)}",
         f"#define COOP_ADDR_MASK {ADDR_MASK:#x}u",
+        "#endif /* COOP_LAYOUT_DEFINED */",
+        "/* with COOP_LAYOUT_ONLY defined, the header stops after the layout */",
+        "#ifndef COOP_LAYOUT_ONLY",
+        "#ifndef COOP_CODE_DEFINED",
+        "#define COOP_CODE_DEFINED",
         f"COOP_DATA uint64_t coop_const_bits[{len(lay.consts)}] = {{{consts}}};",
-    ] + tabs + funcs
+    ] + tabs + funcs + ["#endif /* COOP_CODE_DEFINED */", "#endif /* COOP_LAYOUT_ONLY */"]
     text = "\n".join(hdr) + "\n"
     os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
     tmp = out_path + ".tmp"

@@ -57,8 +57,8 @@ def main():
     init = wr.make_collapse_states(n)
     with wr.Context(n) as ctx:
         ctx.set_queue_launch(args.threads, args.per_cu, args.solo)
-        if args.coop >= 0 and hasattr(ctx, "set_coop_cells"):
-            ctx.set_coop_cells(args.coop)
+        if args.coop >= 0:
+            ctx.set_coop(args.coop, args.coop)
         ctx.upload(init)
         info = ctx.run(perturb=True, max_steps=args.max_outer)
     print(f"run: {info['completed_global_steps']} outer steps, {info['cell_advances']} "

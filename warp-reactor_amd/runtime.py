@@ -75,6 +75,8 @@ _SIGS = {
     "wr_download_cells": (_I, [_VP, _VP, _I]),
     "wr_run": (_I, [_VP, _I, _I, _I, _D]),
     "wr_run_from": (_I, [_VP, _I, _I, _I, _D, _I]),
+    "wr_set_coop": (_I, [_VP, _I, _I]),
+    "wr_get_coop": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "wr_run_info_get": (_I, [_VP, ctypes.POINTER(RunInfo)]),
     "wr_prepare": (_I, [_VP, _I, _D, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
@@ -299,6 +301,19 @@ class Context:
         """Launch shape of the queue-fed kernel (results do not depend on it)."""
         _check(self._lib.wr_set_queue_launch(self._h, int(threads_per_block), int(blocks_per_cu),
                                              int(solo_lanes)), "wr_set_queue_launch")
+
+    def set_coop(self, cells, waves=None):
+        """Advance the `cells` most expensive cells of each step with whole
+        waves (wave 0 of the first `waves` blocks; default waves = cells);
+        0 turns it off.  Results are bit-identical."""
+        waves = cells if waves is None else waves
+        _check(self._lib.wr_set_coop(self._h, int(cells), int(waves)), "wr_set_coop")
+
+    def coop(self):
+        """(cells, waves, LDS bytes per cooperative wave)."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _check(self._lib.wr_get_coop(self._h, *[ctypes.byref(x) for x in v]), "wr_get_coop")
+        return tuple(x.value for x in v)
 
     def queue_launch(self):
         """(threads per block, blocks per CU, solo lanes, grid blocks last used)."""
