@@ -166,6 +166,23 @@ int wr_dt_history(const wr_context *ctx, double *out, int max_entries, int *coun
  * the lowest cell index holding the minimum (or -1). */
 int wr_prepare(wr_context *ctx, int completed_global_steps, double grid_time, int step,
                int perturb, double *dt_grid, int *limiting_cell);
+/* No counterpart in the reference: wr_prepare for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The prepare phase of one outer step on
+ * the caller's DEVICE array of CollapseState records (the layout of the
+ * reference's prepare_grid_timestep_kernel, reproducer.cpp:7696-7734) and the
+ * grid-wide (dt, lowest cell) minimum (7888-7893), enqueued on `stream` (a
+ * hipStream_t; NULL: the default stream) and not waited for.  Results go to
+ * caller-given DEVICE memory: *dt_grid (DBL_MAX when no cell offers a
+ * candidate or a cell failed), *limiting_cell (-1 then) and *failure (1 or
+ * the first failure code); they equal wr_prepare's on the same cells.  The
+ * call allocates nothing: its per-block partials and failure word live in a
+ * fixed device array of the library (768 KiB per GPU), so at most 2^24 cells
+ * (larger counts return WR_ERR_ARGUMENT), and calls that overlap on one GPU
+ * race on it. */
+int wr_prepare_cells_noalloc(wr_collapse_state *cells, int num_cells, int completed_global_steps,
+                             double grid_time, int step, int perturb, double *dt_grid,
+                             int *limiting_cell, int *failure, void *stream);
 /* advance (reproducer.cpp:7736); *integrated = number of cells integrated. */
 int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_time,
                double dt_grid, int *integrated);

@@ -74,13 +74,15 @@ def build_library(extra=()):
     io_obj = os.path.join(objdir, "wr_io.o")
     k_obj = os.path.join(objdir, "wr_kernels.o")
     a_obj = os.path.join(objdir, "wr_adjoint.o")
+    n_obj = os.path.join(objdir, "wr_noalloc.o")
     run(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unknown-pragmas",
          "-c", "-o", io_obj, os.path.join(CSRC, "wr_io.cpp")])
     # the two HIP translation units compile concurrently
     run_parallel([[HIPCC, *HIP_FLAGS, *extra, "-c", "-o", k_obj, os.path.join(CSRC, "wr_kernels.hip")],
-                  [HIPCC, *HIP_FLAGS, *extra, "-c", "-o", a_obj, os.path.join(CSRC, "wr_adjoint.hip")]])
+                  [HIPCC, *HIP_FLAGS, *extra, "-c", "-o", a_obj, os.path.join(CSRC, "wr_adjoint.hip")],
+                  [HIPCC, *HIP_FLAGS, *extra, "-c", "-o", n_obj, os.path.join(CSRC, "wr_noalloc.hip")]])
     tmp = LIB + ".tmp"
-    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, k_obj, a_obj, io_obj])
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, k_obj, a_obj, n_obj, io_obj])
     os.replace(tmp, LIB)
     return LIB
 

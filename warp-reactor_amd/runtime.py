@@ -80,6 +80,7 @@ _SIGS = {
     "wr_run_info_get": (_I, [_VP, ctypes.POINTER(RunInfo)]),
     "wr_prepare": (_I, [_VP, _I, _D, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_I)]),
     "wr_advance": (_I, [_VP, _I, _D, _D, ctypes.POINTER(_I)]),
+    "wr_prepare_cells_noalloc": (_I, [_VP, _I, _I, _D, _I, _I, _VP, _VP, _VP, _VP]),
     "wr_dt_history": (_I, [_VP, _VP, _I, ctypes.POINTER(_I)]),
     "wr_set_advance_kernel": (_I, [_VP, _I]),
     "wr_set_policy": (_I, [_VP, _I]),
