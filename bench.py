@@ -85,10 +85,14 @@ def live_traffic(grid, max_outer, shape, kernel_name="k_advance_q", timeout=300)
                 print(f"# traffic: no {counter} rows for the copy or {kernel_name}",
                       file=sys.stderr, flush=True)
                 return None
-            # copy: 1 GiB read and 1 GiB written per dispatch
-            scale = (1 << 20) / copy[0][1]
+            # the runtime splits each 1 GiB copy into several dispatches: the
+            # scale is the bytes the copies read (or wrote), 3 GiB, over what
+            # the counter summed over all their dispatches
+            copy_kib = sum(nd * mean for nd, mean in copy)
+            scale = 3.0 * (1 << 20) / copy_kib
             out[counter] = {"dispatches": kern[0][0], "kib_per_dispatch_raw": kern[0][1],
-                            "copy_kib_raw": copy[0][1], "scale": scale,
+                            "copy_dispatches": sum(nd for nd, _ in copy),
+                            "copy_kib_total_raw": copy_kib, "scale": scale,
                             "bytes_per_dispatch": kern[0][1] * scale * 1024.0}
     except subprocess.TimeoutExpired:
         print("# traffic: rocprofv3 pass timed out", file=sys.stderr, flush=True)

@@ -20,11 +20,13 @@ def test_help_prints_usage_and_succeeds():
     assert r.returncode == 0
     assert r.stderr.startswith("usage: ")
     assert "--grid N" in r.stderr and "--compare-final-state FILE" in r.stderr
+    assert "--write-final-state FILE" in r.stderr
     assert r.stdout == ""
 
 
 @pytest.mark.parametrize("args", [["--grid"], ["--grid", "0"], ["--grid", "-3"], ["--grid", "x"],
-                                  ["--bogus"], ["--compare-final-state"]])
+                                  ["--bogus"], ["--compare-final-state"],
+                                  ["--write-final-state"], ["--write-final-state", ""]])
 def test_bad_arguments_fail_with_usage(args, tmp_path):
     r = run(*args, cwd=tmp_path)
     assert r.returncode == 1

@@ -118,3 +118,26 @@ def test_cli_legacy_local_policy(tmp_path, wr, oracle_wr):
     assert lines[-1] == "PASS"
     data = (tmp_path / "final_states_grid2_hip_legacy_local.bin").read_bytes()
     assert data == wr.pack_final_states(ocells, 2).tobytes()
+
+
+def test_write_final_state_file_and_manifest(tmp_path):
+    """--write-final-state FILE (the Mojo front end's option,
+    reproducer.mojo:3318-3340, 3529-3535): the packed records go to FILE --
+    the same bytes as the default final_states_grid<N>_hip.bin -- with a
+    FILE.manifest sidecar, and an existing FILE or manifest is never
+    overwritten."""
+    out = tmp_path / "states.bin"
+    r = run(["--grid", "2", "--no-compare-final-state", "--write-final-state", str(out)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert f"final-state output: {out} (8 packed records" in r.stdout
+    assert out.read_bytes() == (tmp_path / "final_states_grid2_hip.bin").read_bytes()
+    manifest = dict(l.split("=", 1) for l in (tmp_path / "states.bin.manifest").read_text().splitlines())
+    assert manifest["format"] == "packed-final-state-v1" and manifest["cell_count"] == "8"
+    assert manifest["policy"] == "gridwide-independent" and manifest["grid_dim"] == "2"
+    assert manifest["network_source_sha256"] == \
+        "be2adf68ec04b3f7854f7d87a173e0f346ea8fea49987cc5412c525db9416f98"
+    before = out.read_bytes()
+    r = run(["--grid", "2", "--no-compare-final-state", "--write-final-state", str(out)], tmp_path)
+    assert r.returncode == 1
+    assert "refusing to overwrite final-state output" in r.stderr
+    assert out.read_bytes() == before

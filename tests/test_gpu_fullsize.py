@@ -167,3 +167,33 @@ def test_run64_against_reference_full_grid_fixture(run64, wr):
     print(f"64^3 x {info['completed_global_steps']} vs reference full-grid run ({len(idx)} sampled "
           f"records, sha256 {str(fx['sha256'])[:16]}): non-D max {np.delete(rel, DEUTERIUM, axis=1).max():.2e}, "
           f"T max {_rel(got['T'], ref['T']).max():.2e}, time equal {np.mean(got['time'] == ref['time']):.3f}")
+
+
+def test_lognormal128_sample_bit_exact_against_oracle(wr, oracle_wr):
+    """The 128^3 log-normal recipe (2,097,152 cells; n_H = exp(z) cm^-3, T
+    log-uniform in [100, 8000] K, seed 1: substep counts per cell differ
+    10-100x) for 40 outer steps (crossing the perturbation at 20): a strided
+    sample of cells replayed on the oracle through the run's dt sequence is
+    bit-identical, all seven ROS2S counters included."""
+    n = 128 ** 3
+    seed, sigma, t0, t1 = wr.RECIPES["lognormal128"]
+    init = wr.make_recipe_states(n, seed, sigma, t0, t1)
+    with wr.Context(n) as ctx:
+        ctx.upload(init)
+        info = ctx.run(perturb=True, max_steps=40)
+        cells = np.zeros(n, dtype=wr.CELL_DTYPE)
+        ctx.download(cells)
+        dts = ctx.dt_history()
+    assert info["status"] == 1 and len(dts) == info["completed_global_steps"] == 40
+    idx = _sample(n, 256)
+    replay = init[idx].copy()
+    assert np.all(oracle_wr.replay_cells(replay, idx, dts, perturb=True) == 1)
+    mask = np.ones(216, bool)
+    mask[156:160] = False
+    got = cells[idx].view(np.uint8).reshape(len(idx), 216)[:, mask]
+    want = replay.view(np.uint8).reshape(len(idx), 216)[:, mask]
+    bad = np.nonzero(np.any(got != want, axis=1))[0]
+    att = cells["stats"]["internal_steps"]
+    print(f"128^3 lognormal x 40: attempts per cell min/median/max {att.min()}/{int(np.median(att))}/"
+          f"{att.max()}, {len(idx)} sampled cells, {len(bad)} differ")
+    assert len(bad) == 0

@@ -87,10 +87,20 @@ def build_library(extra=()):
     return LIB
 
 
+def network_sha256():
+    """The sha256 of the network source the committed IR was imported from."""
+    with open(os.path.join(GEN, "primordial_network.ir")) as f:
+        for line in f:
+            if line.startswith("# source-sha256 "):
+                return line.split()[2]
+    return "unknown"
+
+
 def build_cli():
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
     tmp = CLI + ".tmp"
-    run(["g++", "-std=c++17", "-O2", "-Wall", "-o", tmp, os.path.join(CSRC, "wr_cli.cpp"),
+    run(["g++", "-std=c++17", "-O2", "-Wall", f'-DWR_NETWORK_SHA256="{network_sha256()}"',
+         "-o", tmp, os.path.join(CSRC, "wr_cli.cpp"),
          f"-I{os.path.join(REPO, 'include')}", f"-L{os.path.dirname(LIB)}", "-lwarp_reactor",
          "-Wl,-rpath,$ORIGIN/../lib"])
     os.replace(tmp, CLI)

@@ -7,10 +7,15 @@
 //   wr_reproducer [--grid N] [--perturb|--no-perturb]
 //                 [--compare-final-state FILE|--no-compare-final-state]
 //                 [--driver-policy gridwide-independent|legacy-local]
+//                 [--write-final-state FILE]
 //
 // Writes final_states_grid<N>_hip.bin (_hip_legacy_local.bin for the
 // legacy-local policy; 172-byte records) for N > 1, prints the final state
-// for N = 1, exit status 0 on success and 1 on failure.
+// for N = 1, exit status 0 on success and 1 on failure.  --write-final-state
+// FILE (the option of the reference's Mojo front end, reproducer.mojo:
+// 3283-3340, 3529-3535) also writes the packed records to FILE and a sidecar
+// FILE.manifest naming the record format, cell count, driver policy and
+// network source, and refuses to overwrite either file.
 #include <errno.h>
 #include <limits.h>
 #include <stdio.h>
@@ -36,14 +41,65 @@ struct Options {
     bool show_help = false;
     bool legacy_local = false;
     std::string compare_path;
+    std::string write_path;
 };
+
+#ifndef WR_NETWORK_SHA256
+#define WR_NETWORK_SHA256 "unknown"
+#endif
 
 void print_usage(const char* program) {
     fprintf(stderr,
             "usage: %s [--grid N] [--perturb|--no-perturb]"
             " [--compare-final-state FILE|--no-compare-final-state]"
-            " [--driver-policy gridwide-independent|legacy-local]\n",
+            " [--driver-policy gridwide-independent|legacy-local]"
+            " [--write-final-state FILE]\n",
             program);
+}
+
+bool file_exists(const std::string& path) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    fclose(f);
+    return true;
+}
+
+// The Mojo front end's final-state output (reproducer.mojo:3318-3340): the
+// packed records in cell order, never over an existing file, and a sidecar
+// manifest of key=value lines.
+bool write_final_state_file(const std::vector<wr_collapse_state>& cells, int grid_dim,
+                            const std::string& path, bool legacy_local) {
+    const std::string manifest = path + ".manifest";
+    if (file_exists(path) || file_exists(manifest)) {
+        fprintf(stderr, "refusing to overwrite final-state output: %s\n", path.c_str());
+        return false;
+    }
+    const int n = static_cast<int>(cells.size());
+    std::vector<wr_packed_final_state> packed(cells.size());
+    if (wr_pack_final_states(cells.data(), n, grid_dim, packed.data()) != WR_SUCCESS) {
+        fprintf(stderr, "%s\n", wr_last_error());
+        return false;
+    }
+    FILE* f = fopen(path.c_str(), "wb");
+    const size_t bytes = packed.size() * sizeof(wr_packed_final_state);
+    if (!f || fwrite(packed.data(), 1, bytes, f) != bytes || fclose(f) != 0) {
+        fprintf(stderr, "failed to write final-state output: %s\n", path.c_str());
+        return false;
+    }
+    FILE* m = fopen(manifest.c_str(), "w");
+    if (!m) {
+        fprintf(stderr, "failed to write final-state manifest: %s\n", manifest.c_str());
+        return false;
+    }
+    fprintf(m,
+            "format=packed-final-state-v1\ncell_count=%d\ngrid_dim=%d\npolicy=%s\n"
+            "network_source_sha256=%s\nbackend=%s-gfx950\nabi=%d\n",
+            n, grid_dim, legacy_local ? "legacy-local" : "gridwide-independent",
+            WR_NETWORK_SHA256, kBackend, wr_abi_version());
+    if (fclose(m) != 0) return false;
+    printf("final-state output: %s (%d packed records, manifest %s)\n", path.c_str(), n,
+           manifest.c_str());
+    return true;
 }
 
 bool parse_positive_int(const char* text, int& value) {
@@ -93,6 +149,14 @@ bool parse_args(int argc, char** argv, Options& o) {
                 return false;
             }
             o.legacy_local = v == "legacy-local";
+            continue;
+        }
+        if (arg == "--write-final-state") {
+            if (i + 1 >= argc || argv[i + 1][0] == '\0') {
+                print_usage(argv[0]);
+                return false;
+            }
+            o.write_path = argv[++i];
             continue;
         }
         if (arg == "--no-compare-final-state") {
@@ -282,6 +346,9 @@ int main(int argc, char** argv) {
         }
     }
 
+    if (!o.write_path.empty() &&
+        !write_final_state_file(cells, o.grid_dim, o.write_path, o.legacy_local))
+        return finish(false, comparison_done);
     if (o.grid_dim > 1) {
         const std::string file = "final_states_grid" + std::to_string(o.grid_dim) + "_" + kBackend +
                                  (o.legacy_local ? "_legacy_local" : "") + ".bin";

@@ -396,11 +396,20 @@ def _function_lines(text, name):
     if start is None:
         raise SyntaxError(f"function {name} not found")
     body = []
+    pending, depth = "", 0
     for line in lines[start + 1:]:
-        if line and not line.startswith(" "):
+        if line and not line.startswith(" ") and depth == 0:
             break
-        if line.strip():
-            body.append((start + 1 + len(body), line.strip()))
+        if not line.strip():
+            continue
+        # a statement continues on the next lines while its parentheses are open
+        pending = (pending + " " + line.strip()) if pending else line.strip()
+        depth += line.count("(") - line.count(")")
+        if depth <= 0:
+            body.append((start + 1 + len(body), pending))
+            pending, depth = "", 0
+    if pending:
+        raise SyntaxError(f"unbalanced parentheses in {name}: {pending[:80]}")
     return start + 1, body
 
 

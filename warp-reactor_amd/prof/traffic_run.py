@@ -6,9 +6,9 @@ build and launch shape as the line it reports:
         python3 warp-reactor_amd/prof/traffic_run.py --grid 64 --max-outer 1000 \
             --threads 256 --per-cu 0 --solo 0
 
-First three device-to-device copies of 1 GiB (hipMemcpy, the runtime's copy
-kernel: 1 GiB read and 1 GiB written per dispatch) calibrate what the counters
-read for a known number of bytes; then one collapse run of the workload on
+First three device-to-device copies of 1 GiB (hipMemcpy: the runtime's copy
+kernel, 3 GiB read and 3 GiB written over its dispatches) calibrate what the
+counters read for a known number of bytes; then one collapse run of the workload on
 the library (make_collapse_state cells, perturbations, the given outer steps,
 the queue-fed advance kernel with the given launch shape, HIP graphs).
 Needs no torch."""
