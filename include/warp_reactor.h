@@ -287,8 +287,10 @@ int wr_last_advance_timing(const wr_context *ctx, double *total_ms, int *launche
  * at a recorded run: dt_grid of every outer step and the step sizes of every
  * cell's accepted ROS2S attempts (the forward path is reproducer.cpp:7736-7785,
  * 7205-7347).  Parameters: each cell's state at its first advance and the rate
- * multipliers M_r = exp(theta_r) of the network's reaction flux terms (the
- * species-equation terms of the generated RHS; wr_adjoint_ntheta() of them).
+ * multipliers M_r = exp(theta_r) of the network's reactions (recovered by the
+ * code generator from the reference's rate code; wr_adjoint_ntheta() of them):
+ * M_r scales reaction r in the species rates, in the energy rate where the
+ * network books its heat, and in the Jacobian.
  * States are [18][n] host arrays, field-major: xn[0..13], T, e, rho,
  * density_driver.  e of the first state does not enter the map (eos_rt
  * recomputes it from T).  Perturbations are not part of the map. */

@@ -18,8 +18,9 @@
  *     y' = y + b1 k1 + b2 k2 + b3 k3
  * Its tangent uses dk = E^-1 (dr + dJ k) and its adjoint r_bar = E^-T k_bar,
  * J_bar += r_bar k^T; f's and J's own tangents/adjoints and those of pre/post
- * are generated from the network DAG (warp-reactor_amd/gen/ad.py).  f carries
- * the rate multipliers M_r = exp(theta_r) of the 50 reaction flux terms.
+ * are generated from the network DAG (warp-reactor_amd/gen/ad.py).  f and J
+ * carry the rate multipliers M_r = exp(theta_r) of the network's 45 reactions
+ * (J's tangent and adjoint therefore take M, dM and give M-bar too).
  *
  * Pinning (tests/test_adjoint.py): the record run at M = 1 equals the plain
  * oracle run bit for bit; replay of the recorded steps reproduces the
@@ -110,7 +111,7 @@ static void matvec_add(const double *J, const double *v, double *out) {
 static void ad_substep_tl(const double *y, double h, const double *M, const double *dM,
                           const ad_sub *s, const double *dy, double *dynew) {
     double dJ[NEQ * NEQ], dk1[NEQ], dk2[NEQ], dk3[NEQ], du[NEQ], t[NEQ];
-    ad_jac_jvp(y, dy, dJ);
+    ad_jac_jvp(y, M, dy, dM, dJ);
     ad_f_jvp(y, M, dy, dM, dk1);
     matvec_add(dJ, s->k1, dk1);
     or_lu_solve(s->lu, s->ip, dk1);
@@ -173,8 +174,11 @@ static void ad_substep_adj(const double *y, double h, const double *M, const ad_
     ad_f_vjp(y, M, rb, ub, mb);
     for (int r = 0; r < AD_NTHETA; ++r) Mbar[r] += mb[r];
     for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
-    /* J(y) */
+    /* J(y, M) = J(y) - D(y, M): the reference Jacobian's adjoint, then D's */
     ad_jac_vjp(y, W, ub);
+    for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
+    ad_jacd_vjp(y, M, W, ub, mb);
+    for (int r = 0; r < AD_NTHETA; ++r) Mbar[r] += mb[r];
     for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
 }
 
@@ -374,5 +378,12 @@ void or_adj_f_jvp(const double *y, const double *M, const double *dy, const doub
 void or_adj_f_vjp(const double *y, const double *M, const double *w, double *yb, double *mb) {
     ad_f_vjp(y, M, w, yb, mb);
 }
-void or_adj_jac_jvp(const double *y, const double *dy, double *dJ) { ad_jac_jvp(y, dy, dJ); }
-void or_adj_jac_vjp(const double *y, const double *W, double *yb) { ad_jac_vjp(y, W, yb); }
+void or_adj_jac_jvp(const double *y, const double *M, const double *dy, const double *dM, double *dJ) {
+    ad_jac_jvp(y, M, dy, dM, dJ);
+}
+void or_adj_jac_vjp(const double *y, const double *M, const double *W, double *yb, double *mb) {
+    double ub[NEQ];
+    ad_jac_vjp(y, W, yb);
+    ad_jacd_vjp(y, M, W, ub, mb);
+    for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
+}

@@ -2,6 +2,7 @@
 reference's own CPU code (oracle/_ref/libwr_ref.so).  Both are checkers; the
 product never loads them."""
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -281,14 +282,25 @@ class OracleAdjoint(OracleWrMath):
             "or_adj_fj": (None, [_VP, _VP, _VP, _VP]),
             "or_adj_f_jvp": (None, [_VP, _VP, _VP, _VP, _VP]),
             "or_adj_f_vjp": (None, [_VP, _VP, _VP, _VP, _VP]),
-            "or_adj_jac_jvp": (None, [_VP, _VP, _VP]),
-            "or_adj_jac_vjp": (None, [_VP, _VP, _VP]),
+            "or_adj_jac_jvp": (None, [_VP, _VP, _VP, _VP, _VP]),
+            "or_adj_jac_vjp": (None, [_VP, _VP, _VP, _VP, _VP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
         self.R = L.or_adj_ntheta()
+        table = os.path.join(REPO, "oracle", "_build", "chem_adjoint.json")
+        with open(table) as fh:
+            self._reactions = json.load(fh)["reactions"]
+        if len(self._reactions) != self.R:
+            raise RuntimeError(f"{table} does not describe {path} (make -C oracle)")
+
+    def reactions(self):
+        """The reaction of each multiplier (name, stoichiometry nu over the 14
+        species, rate terms, energy-rate monomials), as the generator that
+        wrote the derivative code recovered them."""
+        return self._reactions
 
     def lu_solve_t(self, lu, ip, x):
         x = np.array(x, dtype=np.float64).copy()
@@ -320,17 +332,17 @@ class OracleAdjoint(OracleWrMath):
                               _p(yb), _p(mb))
         return yb, mb
 
-    def jac_jvp(self, y, dy):
+    def jac_jvp(self, y, M, dy, dM):
         out = np.zeros(225)
-        self.lib.or_adj_jac_jvp(_p(np.ascontiguousarray(y, dtype=np.float64)),
-                                _p(np.ascontiguousarray(dy, dtype=np.float64)), _p(out))
+        self.lib.or_adj_jac_jvp(*[_p(np.ascontiguousarray(a, dtype=np.float64))
+                                  for a in (y, M, dy, dM)], _p(out))
         return out
 
-    def jac_vjp(self, y, W):
-        out = np.zeros(15)
-        self.lib.or_adj_jac_vjp(_p(np.ascontiguousarray(y, dtype=np.float64)),
-                                _p(np.ascontiguousarray(W, dtype=np.float64)), _p(out))
-        return out
+    def jac_vjp(self, y, M, W):
+        yb, mb = np.zeros(15), np.zeros(self.R)
+        self.lib.or_adj_jac_vjp(*[_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (y, M, W)],
+                                _p(yb), _p(mb))
+        return yb, mb
 
     def record(self, cells, nsteps, M, hcap=512):
         """Forward run with multipliers M recording the tape; returns a dict."""

@@ -5,6 +5,14 @@ mathematics fixes rather than by its own formulas:
   * the generated forward code with every rate multiplier 1 is the network
     the reference-pinned oracle evaluates, bit for bit, and the record run
     equals the plain gridwide run;
+  * J_theta is the Jacobian of the reactions it scales (finite differences of
+    each reaction's own part of f);
+  * the multipliers scale whole reactions: each reaction of the recovered
+    table balances H, D, He nuclei and charge, the species rates conserve
+    them and mass under any multipliers, one multiplier moves the rates along
+    its reaction's stoichiometry, and the energy rate carries the H2
+    formation / dissociation heat (4.48, 3.53, 1.83 eV) of exactly the
+    reactions that have one;
   * replaying the recorded steps reproduces the recorded run bit for bit;
   * tangents agree with central finite differences (delta = 1e-6, 8 seeded
     directions over initial states and log-rate multipliers);
@@ -44,17 +52,60 @@ def test_generated_forward_is_the_network(adj, wr):
         assert np.array_equal(J.view(np.uint64), adj.jac(y).ravel().view(np.uint64))
 
 
-def test_rate_multipliers_scale_reaction_fluxes(adj, wr):
-    """f is affine in each multiplier and the flux terms conserve what the
-    reactions conserve: charge (e- and ion charges) of the species rates is
-    zero for any multipliers, as it is for the network."""
-    charge = np.array([-1, 1, 0, -1, 1, 0, 1, -1, 0, 1, 0, 2, 1, 0], dtype=float)
+# Per species: hydrogen nuclei, deuterium nuclei, helium nuclei, charge.
+COMPOSITION = np.array([[0, 0, 0, -1], [1, 0, 0, 1], [1, 0, 0, 0], [1, 0, 0, -1],
+                        [0, 1, 0, 1], [0, 1, 0, 0], [2, 0, 0, 1], [0, 1, 0, -1],
+                        [2, 0, 0, 0], [1, 1, 0, 1], [1, 1, 0, 0], [0, 0, 1, 2],
+                        [0, 0, 1, 1], [0, 0, 1, 0]], dtype=float)
+# The network's species masses (reproducer.cpp:233-252).
+MASS = np.array([9.10938188e-28, 1.67262158e-24, 1.67353251819e-24, 1.67444345638e-24,
+                 3.34512158e-24, 3.34603251819e-24, 3.34615409819e-24, 3.34694345638e-24,
+                 3.34706503638e-24, 5.01865409819e-24, 5.01956503638e-24, 6.69024316e-24,
+                 6.69115409819e-24, 6.69206503638e-24])
+EV = 1.602176634e-12
+
+
+def _imbalance(f, weights):
+    """|sum_i w_i f_i| / sum_i |w_i f_i| (0 when every term is 0)."""
+    t = f[:14] * weights
+    den = np.sum(np.abs(t))
+    return abs(np.sum(t)) / den if den > 0 else 0.0
+
+
+def test_reaction_table_conserves_elements_and_charge(adj):
+    """Every reaction recovered from the generated rate code is balanced in
+    H, D and He nuclei and in charge, and the table holds the network's
+    reactions by name (formation of H2 through H-, H2+ and three bodies,
+    collisional dissociation, the deuterium and helium chemistry)."""
+    rx = adj.reactions()
+    assert len(rx) == adj.R == 45
+    names = [r["reaction"] for r in rx]
+    for r in rx:
+        nu = np.array(r["nu"], dtype=float)
+        assert np.any(nu != 0)
+        assert np.all(COMPOSITION.T @ nu == 0), r
+    for want in ["H + H- -> e- + H2", "H + H2+ -> H+ + H2", "H + H + H -> H + H2",
+                 "H + H + H2 -> H2 + H2", "e- + H2 -> e- + H + H", "H2 + H2 -> H + H + H2",
+                 "e- + H+ -> H", "e- + H -> e- + e- + H+", "e- + He+ -> He", "H + D -> HD",
+                 "D+ + H2 -> H+ + HD", "H+ + HD -> D+ + H2"]:
+        assert want in names, want
+
+
+def test_rate_multipliers_scale_whole_reactions(adj, wr):
+    """f is affine in each multiplier; raising one multiplier changes the 14
+    species rates along that reaction's stoichiometry only, and under any
+    multipliers the species rates conserve H, D and He nuclei, charge and mass
+    as the network does (the multipliers scale whole reactions, not parts of
+    one)."""
+    rx = adj.reactions()
     rng = np.random.default_rng(7)
-    for y in _states(adj, wr)[:20]:
+    states = _states(adj, wr)
+    for y in states[:20]:
         M = np.exp(rng.normal(0, 1.0, adj.R))
         f = adj.f(y, M)
-        scale = np.max(np.abs(f[:14] * charge)) + 1e-300
-        assert abs(np.sum(f[:14] * charge)) <= 1e-10 * scale
+        for k in range(4):
+            assert _imbalance(f, COMPOSITION[:, k]) <= 1e-10
+        assert _imbalance(f, MASS) <= 1e-10
         r = rng.integers(adj.R)
         M2 = M.copy()
         M2[r] *= 3.0
@@ -62,6 +113,122 @@ def test_rate_multipliers_scale_reaction_fluxes(adj, wr):
         M3[r] *= 5.0
         d2, d3 = adj.f(y, M2) - f, adj.f(y, M3) - f
         assert np.allclose(d3, 2.0 * d2, rtol=1e-9, atol=1e-12 * (np.abs(f).max() + 1e-300))
+    # One reaction alone: its part of f (the derivative in M_r, which sums the
+    # reaction's own monomials and no others) lies along its stoichiometry and
+    # is balanced; raising it 100-fold keeps f balanced.
+    M1 = np.ones(adj.R)
+    zero_y = np.zeros(15)
+    for y in states[:8]:
+        for r in range(adj.R):
+            e = np.zeros(adj.R)
+            e[r] = 1.0
+            d = adj.f_jvp(y, M1, zero_y, e)[:14]
+            nu = np.array(rx[r]["nu"], dtype=float)
+            scale = np.abs(d).max()
+            if scale > 0.0:
+                k = np.argmax(np.abs(nu))
+                assert np.allclose(d, d[k] / nu[k] * nu, rtol=0, atol=1e-12 * scale), (r, d, nu)
+                for w in (COMPOSITION[:, 0], COMPOSITION[:, 1], COMPOSITION[:, 2],
+                          COMPOSITION[:, 3], MASS):
+                    assert _imbalance(d, w) <= 1e-10
+            M = M1.copy()
+            M[r] = 100.0
+            g = adj.f(y, M)
+            for k in range(4):
+                assert _imbalance(g, COMPOSITION[:, k]) <= 1e-10
+            assert _imbalance(g, MASS) <= 1e-10
+
+
+def test_thermal_channels_carry_the_reaction_heat(adj, wr):
+    """The energy rate changes with a multiplier only for the reactions whose
+    heat the network books (H2 formation and collisional dissociation); per
+    H2 molecule formed or destroyed that heat is the reaction's energy
+    release (4.48 eV binding energy of H2, 3.53 eV for H- + H, 1.83 eV for
+    H2+ + H), times a density factor in [0, 1] for formation."""
+    rx = adj.reactions()
+    heat = {"H + H- -> e- + H2": 3.53, "H + H2+ -> H+ + H2": 1.83,
+            "H + H + H -> H + H2": 4.48, "H + H + H2 -> H2 + H2": 4.48,
+            "e- + H2 -> e- + H + H": 4.48, "H + H2 -> H + H + H": 4.48,
+            "H2 + H2 -> H + H + H2": 4.48}
+    M1 = np.ones(adj.R)
+    zero_y = np.zeros(15)
+    seen = {}
+    for y in _states(adj, wr):
+        f = adj.f(y, M1)
+        rho = float(np.sum(np.maximum(y[:14], 1e-100) * MASS))
+        for r, entry in enumerate(rx):
+            M = M1.copy()
+            M[r] = 2.0
+            if entry["thermal_monomials"] == 0:
+                assert adj.f(y, M)[14] == f[14], entry["reaction"]
+                continue
+            e = np.zeros(adj.R)
+            e[r] = 1.0
+            d = adj.f_jvp(y, M1, zero_y, e)   # the reaction's own part of f
+            q = heat[entry["reaction"]]
+            if d[8] == 0.0 or d[14] == 0.0:
+                continue
+            per = d[14] * rho / d[8] / EV
+            assert -1e-9 <= per <= q * (1 + 1e-3), (entry["reaction"], per)
+            seen[entry["reaction"]] = max(seen.get(entry["reaction"], 0.0), per)
+    assert {r["reaction"] for r in rx if r["thermal_monomials"]} == set(heat)
+    for name in ("H + H- -> e- + H2", "H + H2+ -> H+ + H2", "H + H + H -> H + H2",
+                 "H + H2 -> H + H + H"):
+        assert abs(seen[name] - heat[name]) <= 1e-3 * heat[name], (name, seen.get(name))
+
+
+GINV = np.array([2.5 if n in (6, 8, 9, 10) else 1.5 for n in range(14)])
+
+
+def _heat_capacity(X):
+    """e / T of the network's EOS (reproducer.cpp:311-339) at abundances X."""
+    mp, rgas = 1.67262192595e-24, 6.02214076e+23 * 1.3806490000000002e-16
+    rho = np.sum(X * MASS)
+    sa = np.sum(X) * (mp / rho)
+    sg = np.sum(X * mp / rho * GINV) / sa
+    return sg * rgas * sa
+
+
+def test_jacobian_carries_the_reactions(adj, wr):
+    """J_theta moves with the multipliers as the Jacobian of the reactions it
+    scales: for each reaction r, dJ/dM_r (the tangent code) equals the
+    finite-difference Jacobian of the reaction's own part of f (df/dM_r), in
+    the reference Jacobian's variables -- abundance columns at fixed
+    temperature (e moved with X so that T stays put), the energy column at
+    fixed abundances.  Points where a piecewise rate fit switches branch
+    inside the difference stencil (one-sided differences disagree) are
+    skipped; the tolerance holds the difference quotient's rounding."""
+    M1 = np.ones(adj.R)
+    z0 = np.zeros(15)
+    checked = skipped = 0
+    for y in _states(adj, wr)[::2]:
+        T = y[14] / _heat_capacity(np.maximum(y[:14], 1e-100))
+        for r in range(adj.R):
+            e = np.zeros(adj.R)
+            e[r] = 1.0
+            A = adj.jac_jvp(y, M1, z0, e).reshape(15, 15)
+            g0 = adj.f_jvp(y, M1, z0, e)
+            for j in range(15):
+                if j < 14 and y[j] <= 1e-90:
+                    continue
+                h = 1e-6 * abs(y[j])
+                zp, zm = y.copy(), y.copy()
+                zp[j] += h
+                zm[j] -= h
+                if j < 14:
+                    zp[14] = T * _heat_capacity(np.maximum(zp[:14], 1e-100))
+                    zm[14] = T * _heat_capacity(np.maximum(zm[:14], 1e-100))
+                gp, gm = adj.f_jvp(zp, M1, z0, e), adj.f_jvp(zm, M1, z0, e)
+                noise = 1e-10 * np.abs(g0) / h   # T is held to rounding: steep fits
+                fp, fm = (gp - g0) / h, (g0 - gm) / h
+                if np.any(np.abs(fp - fm) > 1e-2 * np.maximum(np.abs(fp), np.abs(fm)) + 4 * noise):
+                    skipped += 1
+                    continue
+                checked += 1
+                fd = (gp - gm) / (2 * h)
+                tol = 1e-4 * np.abs(A[:, j]) + noise + 1e-280
+                assert np.all(np.abs(fd - A[:, j]) <= tol), (r, j, A[:, j], fd)
+    assert checked > 10000 and skipped < checked // 100, (checked, skipped)
 
 
 def test_generated_tangents_match_finite_differences(adj, wr):
@@ -78,8 +245,9 @@ def test_generated_tangents_match_finite_differences(adj, wr):
         df = adj.f_jvp(y, M, dy, dM)
         fd = (adj.f(y + eps * dy, M + eps * dM) - adj.f(y - eps * dy, M - eps * dM)) / (2 * eps)
         errs.append(np.max(np.abs(df - fd)) / np.max(np.abs(fd)))
-        dJ = adj.jac_jvp(y, dy)
-        fdJ = (adj.fj(y + eps * dy, M)[1] - adj.fj(y - eps * dy, M)[1]) / (2 * eps)
+        dJ = adj.jac_jvp(y, M, dy, dM)
+        fdJ = (adj.fj(y + eps * dy, M + eps * dM)[1] -
+               adj.fj(y - eps * dy, M - eps * dM)[1]) / (2 * eps)
         errs.append(np.max(np.abs(dJ - fdJ)) / np.max(np.abs(fdJ)))
     assert np.median(errs) < 1e-8 and np.max(errs) < 1e-5, (np.median(errs), np.max(errs))
 
@@ -96,9 +264,10 @@ def test_generated_adjoints_are_transposed_tangents(adj, wr):
         rhs = yb @ dy + mb @ dM
         assert abs(lhs - rhs) <= 1e-11 * (np.abs(w) @ np.abs(adj.f_jvp(y, M, dy, dM)) + abs(lhs))
         W = rng.standard_normal(225)
-        dJ = adj.jac_jvp(y, dy)
+        dJ = adj.jac_jvp(y, M, dy, dM)
         lhs = W @ dJ
-        rhs = adj.jac_vjp(y, W) @ dy
+        yb, mb = adj.jac_vjp(y, M, W)
+        rhs = yb @ dy + mb @ dM
         assert abs(lhs - rhs) <= 1e-11 * (np.abs(W) @ np.abs(dJ) + abs(lhs))
 
 

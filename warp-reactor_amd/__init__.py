@@ -17,6 +17,7 @@ from .runtime import (  # noqa: F401
     Context,
     WarpReactorError,
     adjoint_ntheta,
+    reaction_table,
     compare_final_states,
     library,
     library_path,

@@ -1,8 +1,10 @@
 """torch.autograd binding of the discrete adjoint of the collapse advance.
 
-    theta : (R,) log-rate multipliers of the network's reaction flux terms
-            (R = adjoint_ntheta(); M_r = exp(theta_r) scales flux term r in
-            the species equations of the RHS)
+    theta : (R,) log-rate multipliers of the network's reactions
+            (R = adjoint_ntheta() = 45; M_r = exp(theta_r) scales reaction r
+            wherever it enters: the species rates along its stoichiometry, the
+            energy rate where the network books its heat, and the Jacobian;
+            reaction_table() names them)
     s0    : (18, n) state of each cell at its first advance, field-major:
             xn[0..13] (number densities), T, e, rho, density_driver
     ->  sN: (18, n) state after `nsteps` outer steps (no perturbations)

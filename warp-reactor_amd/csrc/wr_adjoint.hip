@@ -59,7 +59,7 @@ constexpr int kWJ = 0, kWLU = 225, kWW = 450, kWV = 675;
 constexpr int kVF = 0, kVK1 = 15, kVK2 = 30, kVK3 = 45, kVU2 = 60, kVU3 = 75, kVDY = 90,
               kVUB = 105, kVLen = 120;
 constexpr int kWMB = kWV + kVLen;        // accumulated dL/dM of the cell
-constexpr int kWMT = kWMB + kR;          // dL/dM of one f adjoint
+constexpr int kWMT = kWMB + kR;          // dL/dM of one f or J adjoint
 constexpr int kWSTG = kWMT + kR;         // staging slots of the generated code
 constexpr int kWsFixed = kWSTG + AD_NSLOT;
 constexpr int kWsSteps = kWSTG + AD_FWD_NSLOT;  // a step-recompute thread's column
@@ -248,11 +248,30 @@ constexpr int kSY = 0, kSLU = 15, kSK1 = 240, kSK2 = 255, kSK3 = 270, kSU2 = 285
               kSPiv = 315, kSub = 316;
 
 // Private memory of a reverse-sweep thread (the hardware interleaves it by lane):
-// W, the y-adjoint output of the generated code, dL/dM of one f adjoint and the
+// W, the y-adjoint output of the generated code, dL/dM of one f or J adjoint and the
 // cell's accumulated dL/dM, and the staging slots.
 constexpr int kPW = 0, kPUB = 225, kPMT = 240, kPMB = 240 + kR, kPSTG = 240 + 2 * kR;
 constexpr int kPRB = kPSTG + AD_NSLOT;  // r-bar of stages 3, 2, 1 (15 each)
 constexpr int kPrivRev = kPRB + 3 * kNeqs;
+
+// dL/dM of one generated adjoint (kPMT) added to the cell's sum (kPMB), 10
+// multipliers per batch of loads.
+__device__ __forceinline__ void add_mult_adjoint(double* p) {
+#pragma unroll
+    for (int r0 = 0; r0 < kR; r0 += 10) {
+        double mt[10], mb[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+            if (r0 + q < kR) {
+                mt[q] = p[kPMT + r0 + q];
+                mb[q] = p[kPMB + r0 + q];
+            }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+            if (r0 + q < kR) p[kPMB + r0 + q] = mb[q] + mt[q];
+    }
+}
 
 // adjoint of one accepted step: yb in/out.  LU, k, u from the stored slot
 // (element e at slot[e * sst]); W, staging and dL/dM in private memory p.
@@ -296,21 +315,7 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
             }
         }
         ad_f_vjp_st(u, 1, M, 1, rb, 1, p + kPSTG, 1, p + kPUB, p + kPMT);
-        // dL/dM of the call added to the cell's sum, 10 multipliers per batch of loads
-#pragma unroll
-        for (int r0 = 0; r0 < kR; r0 += 10) {
-            double mt[10], mb[10];
-#pragma unroll
-            for (int q = 0; q < 10; ++q)
-                if (r0 + q < kR) {
-                    mt[q] = p[kPMT + r0 + q];
-                    mb[q] = p[kPMB + r0 + q];
-                }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int q = 0; q < 10; ++q)
-                if (r0 + q < kR) p[kPMB + r0 + q] = mb[q] + mt[q];
-        }
+        add_mult_adjoint(p);
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) {
             const double ub = p[kPUB + i];
@@ -345,8 +350,12 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
             }
         }
     }
-    // J(y)
+    // J(y, M) = J(y) - D(y, M): the reference Jacobian's adjoint, then D's
     ad_jac_vjp_st(y, 1, p + kPW, 1, p + kPSTG, 1, p + kPUB, nullptr);
+#pragma unroll
+    for (int i = 0; i < kNeqs; ++i) yb[i] += p[kPUB + i];
+    ad_jacd_vjp_st(y, 1, M, 1, p + kPW, 1, p + kPSTG, 1, p + kPUB, p + kPMT);
+    add_mult_adjoint(p);
 #pragma unroll
     for (int i = 0; i < kNeqs; ++i) yb[i] += p[kPUB + i];
 }

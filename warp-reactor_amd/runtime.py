@@ -150,8 +150,24 @@ def _check(code, what):
 
 
 def adjoint_ntheta():
-    """Number of rate multipliers (reaction flux terms) of the adjoint."""
+    """Number of rate multipliers (reactions of the network) of the adjoint."""
     return library().wr_adjoint_ntheta()
+
+
+def reaction_table():
+    """The reaction behind each rate multiplier of the adjoint, as the code
+    generator recovered it from the network's rate code: a list of dicts with
+    "reaction" (e.g. "H + H- -> e- + H2"), "nu" (stoichiometry over the 14
+    species), "rate_terms" and "thermal_monomials" (terms of the energy rate
+    that carry its heat; 0 when the network books none)."""
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "_gen",
+                        "chem_adjoint.json")
+    with open(path) as fh:
+        table = json.load(fh)["reactions"]
+    if len(table) != adjoint_ntheta():
+        raise WarpReactorError(ERR_ARGUMENT, f"{path} does not describe the built library")
+    return table
 
 
 def make_collapse_states(n):
