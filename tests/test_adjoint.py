@@ -353,3 +353,10 @@ def test_adjoint_is_the_transposed_tangent(adj, recorded):
         rhs = np.sum(g0 * v) + mb.sum(axis=1) @ dM
         assert abs(lhs - rhs) <= 1e-11 * (np.sum(np.abs(gN * tl)) + abs(lhs)), (lhs, rhs)
         assert np.all(g0[15] == 0.0)
+
+
+def test_runtime_reaction_table_is_the_oracle_table(adj, wr):
+    """The library names its multipliers with the table the oracle's
+    derivative code was generated with (same generator, same order)."""
+    assert wr.adjoint_ntheta() == adj.R
+    assert wr.reaction_table() == adj.reactions()
