@@ -327,6 +327,10 @@ int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double 
  * budget); max_steps > 0 caps the steps per window (0: the budget alone).
  * Results do not depend on it. */
 int wr_adjoint_set_window(wr_context *ctx, int max_steps);
+/* (new) Cells per 64-lane wave of the backward pass's reverse sweep, 1..64;
+ * 0 (the default) picks one wave per SIMD when the grid has fewer than 64
+ * cells per SIMD (at least 16 cells per wave).  Results do not depend on it. */
+int wr_adjoint_set_lanes(wr_context *ctx, int cells_per_wave);
 /* Device time (HIP events on the context's stream) of the last record,
  * replay or backward call, ms. */
 int wr_adjoint_last_ms(const wr_context *ctx, double *ms);

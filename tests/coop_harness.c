@@ -41,11 +41,39 @@ static inline double coop_sgn(double x, uint32_t w) {
         __VA_ARGS__                       \
     }
 #define COOP_TW(tab, row) (tab)[(row) * 64 + ln]
+#ifdef COOP_TYPE_CHECK
+/* every access records its type (1 double, 2 boolean word) on its 8-byte slot;
+ * an access of the other type than the slot's last one is counted */
+static uint32_t coop_typed(uint32_t a, unsigned char t);
+#define COOP_LD(a) (*(const double *)(coop_smem + coop_typed((a), 1)))
+#define COOP_LDU(a) (*(const uint32_t *)(coop_smem + coop_typed((a), 2)))
+#define COOP_ST(a, v) (*(double *)(coop_smem + coop_typed((a), 1)) = (v))
+#define COOP_STU(a, v) (*(uint32_t *)(coop_smem + coop_typed((a), 2)) = (v))
+#else
 #define COOP_LD(a) (*(const double *)(coop_smem + (a)))
 #define COOP_LDU(a) (*(const uint32_t *)(coop_smem + (a)))
 #define COOP_ST(a, v) (*(double *)(coop_smem + (a)) = (v))
 #define COOP_STU(a, v) (*(uint32_t *)(coop_smem + (a)) = (v))
+#endif
 #include "coop_network.h"
+
+#ifdef COOP_TYPE_CHECK
+static unsigned char coop_tag[(1 << 18) / 8];
+static long coop_conflicts;
+static uint32_t coop_typed(uint32_t a, unsigned char t) {
+    /* the trash slot takes the results of idle lanes of every class; the
+     * constants (which idle lanes of every class read) are written before a
+     * program runs and never stored by one */
+    if (a > COOP_OFF_TRASH) {
+        unsigned char *g = &coop_tag[a / 8];
+        if (*g && *g != t) ++coop_conflicts;
+        *g = t;
+    }
+    return a;
+}
+/* accesses of one 8-byte LDS slot with two types since the library loaded */
+long coop_type_conflicts(void) { return coop_conflicts; }
+#endif
 
 static void coop_load_inputs(double T, const double *X) {
     memcpy(coop_smem + COOP_OFF_CONST, coop_const_bits, sizeof(coop_const_bits));

@@ -177,6 +177,38 @@ def test_programs_stay_inside_the_lds_region():
     assert lay.total <= 64 * 1024  # one cooperative wave per block, 160 KiB per CU
 
 
+def test_lds_slots_hold_one_type(tmp_path):
+    """No LDS slot is read or written both as a double and as a 32-bit
+    boolean word by the emitted programs.  Where a slot's last load of one
+    type was followed by a store of the other type (a slot reused across
+    types), the GPU compiler's type-based alias analysis took the two accesses
+    as independent and was free to move the store above the load; the slot
+    pools are separate so that no such pair exists."""
+    subprocess.run([sys.executable, os.path.join(GEN, "emit.py"), str(tmp_path / "chem_network.h")],
+                   check=True, capture_output=True)
+    subprocess.run([sys.executable, os.path.join(GEN, "coop.py"), str(tmp_path / "coop_network.h")],
+                   check=True, capture_output=True)
+    so = tmp_path / "libcoop_typed.so"
+    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+                    "-fno-strict-aliasing", "-mfma", "-std=c11", "-DCOOP_TYPE_CHECK",
+                    f"-I{tmp_path}", f"-I{os.path.join(REPO, 'warp-reactor_amd', 'csrc')}",
+                    "-o", str(so), os.path.join(REPO, "tests", "coop_harness.c"), "-lm"],
+                   check=True)
+    lib = ctypes.CDLL(str(so))
+    vp, d = ctypes.c_void_p, ctypes.c_double
+    lib.coop_eval_fused.argtypes = [d, vp, vp, vp]
+    lib.coop_eval_rhs.argtypes = [d, vp, vp]
+    lib.coop_type_conflicts.restype = ctypes.c_long
+    rng = np.random.default_rng(5)
+    f, J = np.zeros(15), np.zeros(225)
+    for _ in range(4):
+        X = 10 ** rng.uniform(-12, 0, 14)
+        T = float(10 ** rng.uniform(1.5, 4.5))
+        lib.coop_eval_fused(T, _p(X), _p(f), _p(J))
+        lib.coop_eval_rhs(T, _p(X), _p(f))
+    assert lib.coop_type_conflicts() == 0
+
+
 @pytest.fixture(scope="module")
 def emulation(tmp_path_factory):
     out = tmp_path_factory.mktemp("coop_emul")

@@ -223,3 +223,20 @@ def test_backward_windows_give_the_same_gradients(recorded):
         assert np.array_equal(_bits(g0w), _bits(g0)), w
         assert np.array_equal(_bits(mbw), _bits(mb)), w
     ctx.adjoint_set_window(0)
+
+
+def test_backward_cells_per_wave_give_the_same_gradients(recorded):
+    """The reverse sweep with 64, 32, 17 and 1 cells per wave (and the
+    automatic choice) equals the single-window default bit for bit: which lane
+    of which wave runs a cell does not change its arithmetic."""
+    ctx, _, _, _, tape, _ = recorded
+    rng = np.random.default_rng(26)
+    gN = rng.standard_normal((18, tape["n"]))
+    ctx.adjoint_set_lanes(64)
+    g0, mb = ctx.adjoint_backward(gN)
+    for lanes in (32, 17, 1, 0):
+        ctx.adjoint_set_lanes(lanes)
+        g0l, mbl = ctx.adjoint_backward(gN)
+        assert np.array_equal(_bits(g0l), _bits(g0)), lanes
+        assert np.array_equal(_bits(mbl), _bits(mb)), lanes
+    ctx.adjoint_set_lanes(0)

@@ -80,11 +80,9 @@ def test_coop_failure_path(wr):
     assert np.array_equal(_bytes(a[1][same]), _bytes(b[1][same]))
 
 
-@pytest.mark.skip(reason="the 64^3 x 1000-step run with cooperative waves went silent for 180 s "
-                         "and was killed; its cause is not shown (CPU emulation of the same "
-                         "cells over all 1000 steps is bit-exact and terminates, 48^3 x 300 "
-                         "steps runs and is faster), so this case stays off the GPU")
 def test_coop_64cube_bit_identical_to_single_lane(wr):
+    """The full 64^3 x 1000-step run: final states, dt history and step count
+    with cooperative waves equal those of the single-lane path."""
     n = 64 ** 3
     init = wr.make_collapse_states(n)
     base = _run(wr, init, 1000, 0, 0)

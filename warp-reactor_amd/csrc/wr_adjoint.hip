@@ -444,12 +444,16 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __res
 
 // Phase B, one thread per cell: the window's steps in reverse from the stored
 // substeps.  lam [18][cap] carries dL/d(state) across windows; dL/dM
-// accumulates in the workspace (cleared when `first`).
+// accumulates in the workspace (cleared when `first`).  Each 64-lane wave
+// (one block) takes `lpw` cells on its first lanes: a wave's sweep is bound by
+// the latency of its own serial chain, not by its lane count, so on grids with
+// fewer than 64 cells per SIMD, fewer cells per wave put a wave on every SIMD.
 __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const double* __restrict__ sub,
                                                            double* __restrict__ lamv, int n, int s0,
-                                                           int s1, int first, int zero) {
+                                                           int s1, int first, int zero, int lpw) {
     using C = Ros2sCoef;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (static_cast<int>(threadIdx.x) >= lpw) return;
+    const int c = blockIdx.x * lpw + threadIdx.x;
     if (c >= n) return;
     double* w = a.ws + c;
     const int st = a.cap;
@@ -522,6 +526,7 @@ __global__ void k_adj_mbar(AdjDev a, double* __restrict__ mbar, int n) {
 struct AdjointState {
     int nsteps = 0, hcap = 0, cap = 0, n = 0;
     int window_max = 0;       // most steps per backward window (0: by memory budget)
+    int lanes_per_wave = 0;   // cells per wave of the reverse sweep (0: by grid size)
     std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
@@ -833,6 +838,19 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     if ((st = put_fields(ctx, d_lam, gN, kNF)) != WR_SUCCESS) return st;
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
     const AdjDev d = dev_view(a, a->M);
+    // cells per wave of the reverse sweep: one wave per SIMD (four per CU)
+    // when the grid has fewer than 64 cells per SIMD, at least 16 per wave
+    int cus = 0;
+    WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+           "hipDeviceGetAttribute(CU count)");
+    int lpw = kAdjBlock;
+    if (a->lanes_per_wave > 0) {
+        lpw = a->lanes_per_wave;
+    } else if (cus > 0) {
+        const long long simds = 4LL * cus;
+        lpw = static_cast<int>((n + simds - 1) / simds);
+        lpw = lpw < 16 ? 16 : (lpw > kAdjBlock ? kAdjBlock : lpw);
+    }
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
     int s1 = a->nsteps;
     while (s1 > 0) {
@@ -863,8 +881,9 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
         hipLaunchKernelGGL(k_adj_steps, dim3(blocks_for(nA, kAdjBlock)), dim3(kAdjBlock), 0,
                            ctx->stream, d, a->sub, n, s0, s1, 0);
         WR_HIP(hipGetLastError(), "k_adj_steps launch");
-        hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, kAdjBlock)), dim3(kAdjBlock), 0,
-                           ctx->stream, d, a->sub, d_lam, n, s0, s1, s1 == a->nsteps ? 1 : 0, 0);
+        hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, lpw)), dim3(kAdjBlock), 0,
+                           ctx->stream, d, a->sub, d_lam, n, s0, s1, s1 == a->nsteps ? 1 : 0, 0,
+                           lpw);
         WR_HIP(hipGetLastError(), "k_adj_reverse launch");
         s1 = s0;
     }
@@ -883,6 +902,14 @@ int wr_adjoint_set_window(wr_context* ctx, int max_steps) {
     if (!ctx || max_steps < 0) return fail(WR_ERR_ARGUMENT, "wr_adjoint_set_window: bad argument");
     if (!ctx->adj) ctx->adj = new AdjointState();
     ctx->adj->window_max = max_steps;
+    return WR_SUCCESS;
+}
+
+int wr_adjoint_set_lanes(wr_context* ctx, int cells_per_wave) {
+    if (!ctx || cells_per_wave < 0 || cells_per_wave > kAdjBlock)
+        return fail(WR_ERR_ARGUMENT, "wr_adjoint_set_lanes: cells per wave 0..64");
+    if (!ctx->adj) ctx->adj = new AdjointState();
+    ctx->adj->lanes_per_wave = cells_per_wave;
     return WR_SUCCESS;
 }
 

@@ -43,6 +43,16 @@ __device__ __forceinline__ double coop_sgn(double x, uint32_t w) {
     return __longlong_as_double(b ^ (static_cast<long long>(w & 0x80000000u) << 32));
 }
 
+// Every access to the cooperative LDS region goes through may_alias types:
+// the region is a byte array holding doubles, 32-bit booleans and 64-bit
+// constant patterns, and the order of a load and a later store to the same
+// bytes by other lanes is the programs' only synchronisation.  Plain typed
+// accesses would let type-based alias analysis reorder a 32-bit access and a
+// 64-bit access to the same bytes.
+typedef double __attribute__((may_alias)) coop_f64;
+typedef uint32_t __attribute__((may_alias)) coop_u32;
+typedef uint64_t __attribute__((may_alias)) coop_u64;
+
 #define COOP_FN __device__ __forceinline__
 #define COOP_PARAMS const int lane
 #define COOP_DATA __device__ const
@@ -52,10 +62,10 @@ __device__ __forceinline__ double coop_sgn(double x, uint32_t w) {
 #define COOP_L(...) \
     { __VA_ARGS__ }
 #define COOP_TW(tab, row) (tab)[(row) * 64 + lane]
-#define COOP_LD(a) (*reinterpret_cast<const double*>(wr_coop_lds + (a)))
-#define COOP_LDU(a) (*reinterpret_cast<const uint32_t*>(wr_coop_lds + (a)))
-#define COOP_ST(a, v) (*reinterpret_cast<double*>(wr_coop_lds + (a)) = (v))
-#define COOP_STU(a, v) (*reinterpret_cast<uint32_t*>(wr_coop_lds + (a)) = (v))
+#define COOP_LD(a) (*reinterpret_cast<const coop_f64*>(wr_coop_lds + (a)))
+#define COOP_LDU(a) (*reinterpret_cast<const coop_u32*>(wr_coop_lds + (a)))
+#define COOP_ST(a, v) (*reinterpret_cast<coop_f64*>(wr_coop_lds + (a)) = (v))
+#define COOP_STU(a, v) (*reinterpret_cast<coop_u32*>(wr_coop_lds + (a)) = (v))
 #define COOP_LAYOUT_ONLY
 #include "_gen/coop_network.h"
 #undef COOP_LAYOUT_ONLY
@@ -70,11 +80,11 @@ __shared__ __attribute__((aligned(16))) char wr_coop_lds[COOP_LDS_BYTES];
 #include "_gen/coop_network.h"
 
 // LDS accessors of the hand-written part (byte offsets from the generator).
-__device__ __forceinline__ double& coop_at(int off) {
-    return *reinterpret_cast<double*>(wr_coop_lds + off);
+__device__ __forceinline__ coop_f64& coop_at(int off) {
+    return *reinterpret_cast<coop_f64*>(wr_coop_lds + off);
 }
 constexpr int kCoopVecY = 0, kCoopVecF0 = 15, kCoopVecYnew = 75, kCoopVecDy = 90;
-__device__ __forceinline__ double& coop_vec(int v, int i) { return coop_at(COOP_OFF_VEC + 8 * (v + i)); }
+__device__ __forceinline__ coop_f64& coop_vec(int v, int i) { return coop_at(COOP_OFF_VEC + 8 * (v + i)); }
 
 // A double of lane `src` (wave-uniform) on every lane.
 __device__ __forceinline__ double coop_readlane(double v, int src) {
@@ -399,7 +409,7 @@ __device__ __noinline__ void coop_cell(DevCells s, Ctl* ctl, int c, int lane) {
 // queue positions (the most expensive cells of the last step), one at a time.
 __device__ __forceinline__ void coop_run(DevCells s, Ctl* ctl, int lane) {
     // the programs' constant slots
-    uint64_t* kc = reinterpret_cast<uint64_t*>(wr_coop_lds + COOP_OFF_CONST);
+    coop_u64* kc = reinterpret_cast<coop_u64*>(wr_coop_lds + COOP_OFF_CONST);
     for (int i = lane; i < COOP_NCONST; i += 64) kc[i] = coop_const_bits[i];
     const int ncoop = ctl->coop_cells;
     for (;;) {

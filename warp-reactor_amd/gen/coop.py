@@ -30,7 +30,11 @@ addition of the negated operand, as gfx950 executes every FP64 subtract.
 Memory order is the only synchronisation: one wave owns its LDS region, and
 LDS operations of a wave complete in issue order.  The emitted code issues
 the LDS loads of step k+1 before the store of step k; the scheduler therefore
-never lets a step read a value produced by the step right before it.
+never lets a step read a value produced by the step right before it.  The
+order of a slot's last load and the store that reuses the slot holds only
+while the compiler sees the two accesses as possibly aliasing: booleans and
+numbers live in separate slots (one access type per LDS byte), and the HIP
+accessors use may_alias types.
 
 The emitted header compiles as HIP (one lane per thread) and as C (every
 vector statement looped over 64 lanes), so the CPU test suite checks the
@@ -289,7 +293,7 @@ class Layout:
     """LDS byte layout of a cooperative wave's region (the generator decides
     the constant and value-slot counts; the hand-written part uses the rest)."""
 
-    def __init__(self, consts, nslots):
+    def __init__(self, consts, nnum, nbool):
         self.consts = consts                  # list of 64-bit patterns
         off = 0
         self.const = off
@@ -298,8 +302,10 @@ class Layout:
         off += 8
         self.inp = off                        # T, X[0..13]
         off += 8 * 15
-        self.val = off
-        off += 8 * nslots
+        self.val = off                        # number slots (doubles)
+        off += 8 * nnum
+        self.valb = off                       # boolean slots (32-bit words, 8-byte pitch)
+        off += 8 * nbool
         self.jac = off                        # J[15][15] row-major
         off += 8 * 225
         self.vec = off                        # y, f0, ak1, ak2, ak3, ynew, dy (15 each)
@@ -375,22 +381,34 @@ def assign(prog, outputs, layout_fn):
     for n in in_lds:
         if n not in addr:
             deaths[lds_read[n]].append(n)
-    free, nslots = [], 0
+    # Booleans (32-bit words) and numbers (doubles) draw slots from separate
+    # pools, so no LDS byte is ever accessed with two types.  The HIP
+    # compiler's type-based alias analysis takes a 32-bit load and a 64-bit
+    # store (or the reverse) as independent: with a slot reused across types,
+    # it may move the store of a slot's new value above the last load of its
+    # old value.
+    free = {False: [], True: []}
+    count = {False: 0, True: 0}
     slot = {}
     for k in range(K):
         for n in births.get(k, ()):
-            if free:
-                s = free.pop()
+            b = is_bool(d, n)
+            pool = free[b]
+            if pool:
+                s = pool.pop()
             else:
-                s = nslots
-                nslots += 1
-            slot[n] = s
+                s = count[b]
+                count[b] += 1
+            slot[n] = (b, s)
         for n in deaths.get(k, ()):
             if not NO_REUSE:
-                free.append(slot[n])
-        free.sort(reverse=True)
+                free[slot[n][0]].append(slot[n][1])
+        for pool in free.values():
+            pool.sort(reverse=True)
     prog.slot = slot
-    prog.nslots = nslots
+    prog.nslots_num = count[False]
+    prog.nslots_bool = count[True]
+    prog.nslots = count[False] + count[True]
     prog.outputs_home = addr
     prog.extra = extra
     prog.in_lds = in_lds
@@ -452,7 +470,8 @@ def finalize(prog, lay, outputs_addr):
             return lay.inp + 8 * (1 + node.val)
         if n in prog.outputs_home:
             return outputs_addr(prog.outputs_home[n])
-        return lay.val + 8 * prog.slot[n]
+        b, s = prog.slot[n]
+        return (lay.valb if b else lay.val) + 8 * s
 
     prog.a_of = a_of
     tab = []
@@ -880,7 +899,8 @@ def build_programs(d0):
             outs.setdefault(r, []).append(("J", k))
         assign(prog, outs, None)
         progs.append(prog)
-    lay = Layout(constants_of(progs), max(p.nslots for p in progs))
+    lay = Layout(constants_of(progs), max(p.nslots_num for p in progs),
+                 max(p.nslots_bool for p in progs))
     oaddr = lambda key: OUT_ADDR[key[0]](lay, key[1])
     for prog in progs:
         finalize(prog, lay, oaddr)

@@ -111,6 +111,7 @@ _SIGS = {
     "wr_adjoint_tape": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "wr_adjoint_last_ms": (_I, [_VP, ctypes.POINTER(_D)]),
     "wr_adjoint_set_window": (_I, [_VP, _I]),
+    "wr_adjoint_set_lanes": (_I, [_VP, _I]),
     "wr_make_collapse_states": (_I, [_VP, _I]),
     "wr_make_recipe_states": (_I, [_VP, _I, ctypes.c_uint64, _D, _D, _D]),
     "wr_pack_final_states": (_I, [_VP, _I, _I, _VP]),
@@ -475,6 +476,10 @@ class Context:
     def adjoint_set_window(self, max_steps=0):
         """Cap the steps per backward window (0: by the memory budget)."""
         _check(self._lib.wr_adjoint_set_window(self._h, int(max_steps)), "wr_adjoint_set_window")
+
+    def adjoint_set_lanes(self, cells_per_wave=0):
+        """Cells per wave of the reverse sweep, 1..64 (0: by grid size)."""
+        _check(self._lib.wr_adjoint_set_lanes(self._h, int(cells_per_wave)), "wr_adjoint_set_lanes")
 
     def adjoint_last_ms(self):
         ms = ctypes.c_double()
