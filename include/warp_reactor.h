@@ -225,9 +225,12 @@ int wr_set_queue_launch(wr_context *ctx, int threads_per_block, int blocks_per_c
  * path takes the other cells, and the cooperative waves join it when their
  * cells are done.  Results are bit-identical to the single-lane path.
  * Applies to the queue-fed FP64 kernel with the reference's Jacobian rule
- * and the gridwide policy; cells = 0 or waves = 0 turns it off. */
+ * and the gridwide policy; cells = 0 or waves = 0 turns it off.  cells = -1
+ * (the default) takes one cooperative cell per 2,048 cells of the context,
+ * at most 128, with as many waves (none under 2,048 cells). */
 int wr_set_coop(wr_context *ctx, int cells, int waves);
-/* The current setting and the LDS bytes a cooperative wave uses. */
+/* The cells and waves the setting gives for the context's grid, and the LDS
+ * bytes a cooperative wave uses. */
 int wr_get_coop(const wr_context *ctx, int *cells, int *waves, int *lds_bytes);
 
 /* The current shape and the grid size last used (0 before the first launch). */
@@ -329,7 +332,7 @@ int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double 
 int wr_adjoint_set_window(wr_context *ctx, int max_steps);
 /* (new) Cells per 64-lane wave of the backward pass's reverse sweep, 1..64;
  * 0 (the default) picks one wave per SIMD when the grid has fewer than 64
- * cells per SIMD (at least 16 cells per wave).  Results do not depend on it. */
+ * cells per SIMD (at least 32 cells per wave).  Results do not depend on it. */
 int wr_adjoint_set_lanes(wr_context *ctx, int cells_per_wave);
 /* Device time (HIP events on the context's stream) of the last record,
  * replay or backward call, ms. */

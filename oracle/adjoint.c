@@ -129,6 +129,13 @@ static void ad_substep_tl(const double *y, double h, const double *M, const doub
         dynew[i] = dy[i] + c_b1 * dk1[i] + c_b2 * dk2[i] + c_b3 * dk3[i];
 }
 
+/* Whether every rate multiplier is exactly 1 (theta = 0). */
+static int multipliers_are_one(const double *M) {
+    for (int r = 0; r < AD_NTHETA; ++r)
+        if (M[r] != 1.0) return 0;
+    return 1;
+}
+
 /* adjoint of one substep: yb (in: adjoint of y', out: adjoint of y); Mbar += */
 static void ad_substep_adj(const double *y, double h, const double *M, const ad_sub *s,
                            double *yb, double *Mbar) {
@@ -174,9 +181,17 @@ static void ad_substep_adj(const double *y, double h, const double *M, const ad_
     ad_f_vjp(y, M, rb, ub, mb);
     for (int r = 0; r < AD_NTHETA; ++r) Mbar[r] += mb[r];
     for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
-    /* J(y, M) = J(y) - D(y, M): the reference Jacobian's adjoint, then D's */
+    /* J(y, M) = J(y) - D(y, M): the reference Jacobian's adjoint, then D's.
+     * D = sum_r (1 - M_r) dG_r/dz is linear in M: with every M_r = 1 its
+     * y-adjoint is zero and only its M-adjoint (which does not depend on M)
+     * is computed. */
     ad_jac_vjp(y, W, ub);
     for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
+    if (multipliers_are_one(M)) {
+        ad_jacd_mbar(y, W, mb);
+        for (int r = 0; r < AD_NTHETA; ++r) Mbar[r] += mb[r];
+        return;
+    }
     ad_jacd_vjp(y, M, W, ub, mb);
     for (int r = 0; r < AD_NTHETA; ++r) Mbar[r] += mb[r];
     for (int i = 0; i < NEQ; ++i) yb[i] += ub[i];
@@ -381,6 +396,10 @@ void or_adj_f_vjp(const double *y, const double *M, const double *w, double *yb,
 void or_adj_jac_jvp(const double *y, const double *M, const double *dy, const double *dM, double *dJ) {
     ad_jac_jvp(y, M, dy, dM, dJ);
 }
+void or_adj_jacd_vjp(const double *y, const double *M, const double *W, double *yb, double *mb) {
+    ad_jacd_vjp(y, M, W, yb, mb);
+}
+void or_adj_jacd_mbar(const double *y, const double *W, double *mb) { ad_jacd_mbar(y, W, mb); }
 void or_adj_jac_vjp(const double *y, const double *M, const double *W, double *yb, double *mb) {
     double ub[NEQ];
     ad_jac_vjp(y, W, yb);

@@ -284,6 +284,8 @@ class OracleAdjoint(OracleWrMath):
             "or_adj_f_vjp": (None, [_VP, _VP, _VP, _VP, _VP]),
             "or_adj_jac_jvp": (None, [_VP, _VP, _VP, _VP, _VP]),
             "or_adj_jac_vjp": (None, [_VP, _VP, _VP, _VP, _VP]),
+            "or_adj_jacd_vjp": (None, [_VP, _VP, _VP, _VP, _VP]),
+            "or_adj_jacd_mbar": (None, [_VP, _VP, _VP]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -343,6 +345,20 @@ class OracleAdjoint(OracleWrMath):
         self.lib.or_adj_jac_vjp(*[_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (y, M, W)],
                                 _p(yb), _p(mb))
         return yb, mb
+
+    def jacd_vjp(self, y, M, W):
+        """Adjoint of D(y, M) (J_theta = J - D) with seed W: (ybar, Mbar)."""
+        yb, mb = np.zeros(15), np.zeros(self.R)
+        self.lib.or_adj_jacd_vjp(*[_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (y, M, W)],
+                                 _p(yb), _p(mb))
+        return yb, mb
+
+    def jacd_mbar(self, y, W):
+        """The M-adjoint of D alone (it does not depend on M)."""
+        mb = np.zeros(self.R)
+        self.lib.or_adj_jacd_mbar(*[_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (y, W)],
+                                  _p(mb))
+        return mb
 
     def record(self, cells, nsteps, M, hcap=512):
         """Forward run with multipliers M recording the tape; returns a dict."""

@@ -271,6 +271,26 @@ def test_generated_adjoints_are_transposed_tangents(adj, wr):
         assert abs(lhs - rhs) <= 1e-11 * (np.abs(W) @ np.abs(dJ) + abs(lhs))
 
 
+def test_d_adjoint_at_unit_multipliers(adj, wr):
+    """D = sum_r (1 - M_r) dG_r/dz is linear in M: the M-adjoint of D alone
+    (ad_jacd_mbar, what the backward pass runs when every multiplier is 1)
+    equals the M-adjoint of the full D adjoint bit for bit at any M, and at
+    M = 1 the full adjoint's y-part is exactly zero, so skipping it there
+    changes nothing."""
+    rng = np.random.default_rng(12)
+    ones = np.ones(adj.R)
+    for y in _states(adj, wr):
+        W = rng.standard_normal(225)
+        mb_only = adj.jacd_mbar(y, W)
+        yb1, mb1 = adj.jacd_vjp(y, ones, W)
+        assert np.all(yb1 == 0.0)
+        assert np.array_equal(mb_only.view(np.uint64), mb1.view(np.uint64))
+        M = np.exp(rng.normal(0, 0.3, adj.R))
+        ybM, mbM = adj.jacd_vjp(y, M, W)
+        assert np.array_equal(mb_only.view(np.uint64), mbM.view(np.uint64))
+        assert np.any(ybM != 0.0)
+
+
 def test_transposed_lu_solve(adj):
     rng = np.random.default_rng(10)
     for k in range(20):

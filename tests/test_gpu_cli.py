@@ -141,3 +141,23 @@ def test_write_final_state_file_and_manifest(tmp_path):
     assert r.returncode == 1
     assert "refusing to overwrite final-state output" in r.stderr
     assert out.read_bytes() == before
+
+
+SHIM = os.path.join(REPO, "oracle", "_ref", "ref_shim")
+
+
+@pytest.mark.skipif(not os.path.exists(SHIM), reason="oracle/_ref/ref_shim not built (needs the "
+                    "reference source where build() runs)")
+def test_reference_side_shim_equals_the_cli(tmp_path):
+    """The run_cells_cuda shim of INTEGRATION.md, compiled against the
+    reference's own CollapseState, make_collapse_state and final-state writer
+    (oracle/ref_shim.cpp), drives the library through wr_run_cells and writes
+    the same bytes as the library's CLI for the same grid."""
+    out = tmp_path / "shim_grid4.bin"
+    r = subprocess.run([SHIM, "4", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    c = run(["--grid", "4", "--no-compare-final-state"], tmp_path)
+    assert c.returncode == 0, c.stderr
+    steps = next(l for l in c.stdout.splitlines() if l.startswith("completed global"))
+    assert steps in r.stdout.splitlines()
+    assert out.read_bytes() == (tmp_path / "final_states_grid4_hip.bin").read_bytes()

@@ -73,6 +73,7 @@ struct AdjDev {
     double* ws;      // [kWsFixed][cap]
     int* err;        // first error: -100 a cell did not integrate, -101 tape full
     int hcap, nsteps, cap;
+    bool unit;       // every multiplier of M is exactly 1
 };
 
 __device__ __forceinline__ void adj_error(int* err, int code) { atomicCAS(err, 0, code); }
@@ -278,8 +279,9 @@ __device__ __forceinline__ void add_mult_adjoint(double* p) {
 // Stages 3, 2, 1 run through one loop body, so each generated adjoint is
 // instantiated once.
 __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], double h,
-                                                   const double* M, const double* slot, int sst,
-                                                   double* p, uint64_t piv, double (&yb)[kNeqs]) {
+                                                   const double* M, bool unit, const double* slot,
+                                                   int sst, double* p, uint64_t piv,
+                                                   double (&yb)[kNeqs]) {
     using C = Ros2sCoef;
     const double* LU = slot + kSLU * sst;
     double kb1[kNeqs], kb2[kNeqs], kb3[kNeqs];
@@ -350,10 +352,17 @@ __device__ __forceinline__ void adj_substep_reverse(const double (&y)[kNeqs], do
             }
         }
     }
-    // J(y, M) = J(y) - D(y, M): the reference Jacobian's adjoint, then D's
+    // J(y, M) = J(y) - D(y, M): the reference Jacobian's adjoint, then D's.
+    // D = sum_r (1 - M_r) dG_r/dz is linear in M: with every M_r = 1 (`unit`)
+    // its y-adjoint is zero and only its M-adjoint (independent of M) runs.
     ad_jac_vjp_st(y, 1, p + kPW, 1, p + kPSTG, 1, p + kPUB, nullptr);
 #pragma unroll
     for (int i = 0; i < kNeqs; ++i) yb[i] += p[kPUB + i];
+    if (unit) {
+        ad_jacd_mbar_st(y, 1, p + kPW, 1, p + kPSTG, 1, nullptr, p + kPMT);
+        add_mult_adjoint(p);
+        return;
+    }
     ad_jacd_vjp_st(y, 1, M, 1, p + kPW, 1, p + kPSTG, 1, p + kPUB, p + kPMT);
     add_mult_adjoint(p);
 #pragma unroll
@@ -497,8 +506,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const doubl
             for (int i = 0; i < kNeqs; ++i) yk[i] = slot[(kSY + i) * a.cap];
             const uint64_t piv =
                 static_cast<uint64_t>(__double_as_longlong(slot[kSPiv * a.cap]));
-            adj_substep_reverse(yk, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, slot, a.cap,
-                                p, piv, yb);
+            adj_substep_reverse(yk, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, a.unit, slot,
+                                a.cap, p, piv, yb);
         }
         // density update and eos_rt
         const double ws2[2] = {rho1b, lam[17]};
@@ -539,6 +548,7 @@ struct AdjointState {
     double* htape = nullptr;
     double* dts = nullptr;
     double* M = nullptr;   // multipliers of the recording
+    bool unit = false;     // every multiplier of the recording is exactly 1
     double* Mr = nullptr;  // multipliers of a replay
     double* ws = nullptr;
     size_t ws_elems = 0;
@@ -586,6 +596,7 @@ AdjDev dev_view(const AdjointState* a, const double* M) {
     d.hcap = a->hcap;
     d.nsteps = a->nsteps;
     d.cap = a->cap;
+    d.unit = a->unit;
     return d;
 }
 
@@ -679,6 +690,8 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     WR_HIP(hipMemcpyAsync(a->M, multipliers, kR * sizeof(double), hipMemcpyHostToDevice,
                           ctx->stream),
            "hipMemcpy(multipliers)");
+    a->unit = true;
+    for (int r = 0; r < kR; ++r) a->unit = a->unit && multipliers[r] == 1.0;
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
     WR_HIP(hipMemsetAsync(a->hoff, 0, static_cast<size_t>(a->cap) * sizeof(int), ctx->stream),
            "hipMemset");
@@ -839,7 +852,8 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
     const AdjDev d = dev_view(a, a->M);
     // cells per wave of the reverse sweep: one wave per SIMD (four per CU)
-    // when the grid has fewer than 64 cells per SIMD, at least 16 per wave
+    // when the grid has fewer than 64 cells per SIMD, at least 32 per wave
+    // (at 32^3, 16 cells per wave -- two waves per SIMD -- took 1.65x longer)
     int cus = 0;
     WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
            "hipDeviceGetAttribute(CU count)");
@@ -849,7 +863,7 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     } else if (cus > 0) {
         const long long simds = 4LL * cus;
         lpw = static_cast<int>((n + simds - 1) / simds);
-        lpw = lpw < 16 ? 16 : (lpw > kAdjBlock ? kAdjBlock : lpw);
+        lpw = lpw < 32 ? 32 : (lpw > kAdjBlock ? kAdjBlock : lpw);
     }
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
     int s1 = a->nsteps;

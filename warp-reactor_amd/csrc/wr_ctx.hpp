@@ -142,8 +142,9 @@ struct wr_context {
     int q_solo = 64;
     int q_blocks = 0;
     // wave-cooperative advance of the most expensive cells (wr_set_coop):
-    // queue positions given to whole waves, and how many waves take them
-    int coop_cells = 0;
+    // queue positions given to whole waves (-1: by grid size), and how many
+    // waves take them
+    int coop_cells = -1;
     int coop_waves = 0;
     // HIP-event timing of the advance launches of wr_run (when enabled)
     static constexpr int kChunk = 8;

@@ -716,11 +716,29 @@ int wr_download_cells(wr_context* ctx, wr_collapse_state* cells, int num_cells) 
     return WR_SUCCESS;
 }
 
+// Cooperative cells and waves of the context's grid: the setting, or (cells
+// = -1, the default) one cooperative cell per 2,048 cells, at most 128, with
+// as many waves.  On the 64^3 run 8 to 128 cooperative cells all take
+// 2.71-2.81 s against 4.39 s without; grids under 2,048 cells keep every
+// cell on the single-lane path.
+static void coop_resolved(const wr_context* ctx, int* cells, int* waves) {
+    if (ctx->coop_cells >= 0) {
+        *cells = ctx->coop_cells;
+        *waves = ctx->coop_waves;
+        return;
+    }
+    const int c = ctx->n / 2048 < kCoopAutoMax ? ctx->n / 2048 : kCoopAutoMax;
+    *cells = c;
+    *waves = c;
+}
+
 // Whether the wave-cooperative advance applies to subsequent launches: the
 // queue-fed FP64 kernel with the reference's Jacobian rule, gridwide policy.
 static bool coop_active(const wr_context* ctx) {
-    return ctx->coop_cells > 0 && ctx->coop_waves > 0 && ctx->advance_kernel == 1 &&
-           ctx->lu_mode == 0 && ctx->jac_age_max == 1 && ctx->policy == WR_POLICY_GRIDWIDE;
+    int cells = 0, waves = 0;
+    coop_resolved(ctx, &cells, &waves);
+    return cells > 0 && waves > 0 && ctx->advance_kernel == 1 && ctx->lu_mode == 0 &&
+           ctx->jac_age_max == 1 && ctx->policy == WR_POLICY_GRIDWIDE;
 }
 
 // first_step: the outer-loop index of the run's first step, which the
@@ -747,10 +765,7 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.lu_sweeps = ctx->lu_sweeps;
     h.solo_lanes = ctx->q_solo;
     h.jac_age_max = ctx->jac_age_max;
-    if (coop_active(ctx)) {
-        h.coop_cells = ctx->coop_cells;
-        h.coop_waves = ctx->coop_waves;
-    }
+    if (coop_active(ctx)) coop_resolved(ctx, &h.coop_cells, &h.coop_waves);
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
            "hipMemcpy(control to device)");
@@ -1117,8 +1132,8 @@ int wr_set_queue_launch(wr_context* ctx, int threads_per_block, int blocks_per_c
 }
 
 int wr_set_coop(wr_context* ctx, int cells, int waves) {
-    if (!ctx || cells < 0 || waves < 0 || cells > (1 << 20) || waves > 4096)
-        return fail(WR_ERR_ARGUMENT, "wr_set_coop: cells 0..2^20, waves 0..4096");
+    if (!ctx || cells < -1 || waves < 0 || cells > (1 << 20) || waves > 4096)
+        return fail(WR_ERR_ARGUMENT, "wr_set_coop: cells -1..2^20, waves 0..4096");
     ctx->coop_cells = cells;
     ctx->coop_waves = waves;
     ctx->q_blocks = 0;  // occupancy with the cooperative LDS
@@ -1127,8 +1142,7 @@ int wr_set_coop(wr_context* ctx, int cells, int waves) {
 
 int wr_get_coop(const wr_context* ctx, int* cells, int* waves, int* lds_bytes) {
     if (!ctx || !cells || !waves) return fail(WR_ERR_ARGUMENT, "wr_get_coop: bad argument");
-    *cells = ctx->coop_cells;
-    *waves = ctx->coop_waves;
+    coop_resolved(ctx, cells, waves);
     if (lds_bytes) *lds_bytes = COOP_LDS_BYTES;
     return WR_SUCCESS;
 }

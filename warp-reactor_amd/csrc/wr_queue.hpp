@@ -39,6 +39,8 @@ constexpr int kQBlock = 256;
 // tens of attempts then does not share instruction issue and fetch with
 // three other waves while the bulk of the cells is being advanced.
 constexpr int kSoloLanes = 64;  // default of Ctl::solo_lanes
+// Most cooperative cells of the default setting (one per 2,048 cells).
+constexpr int kCoopAutoMax = 128;
 
 // kMixed: the FP32-factorisation form (E = fac I - J factorised in FP32, FP64
 // residual and refinement sweeps per stage solve, FP64 factors when the FP32

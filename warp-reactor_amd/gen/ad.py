@@ -853,6 +853,12 @@ def build(out_path, info_path=None, dev_path=None):
         ("ad_jacd_vjp", f"{P_Y}, const double* M, const double* W, double* ybar, double* Mbar",
          [(f"ybar[{i}]", D_a[i]) for i in range(NEQ)] +
          [(f"Mbar[{r}]", D_a[NEQ + r]) for r in range(R)]),
+        # D is linear in M: its M-adjoint (the same nodes as ad_jacd_vjp's
+        # Mbar) does not depend on M, and its y-adjoint carries the factor
+        # 1 - M_r, so with every M_r = 1 the y-adjoint is zero and this is all
+        # of D's adjoint
+        ("ad_jacd_mbar", f"{P_Y}, const double* W, double* Mbar",
+         [(f"Mbar[{r}]", D_a[NEQ + r]) for r in range(R)]),
         ("ad_pre", "const double* xn, const double* s, double* y0, double* s1",
          [(f"y0[{i}]", m.pre_y[i]) for i in range(NEQ)] +
          [("s1[0]", m.pre_rho), ("s1[1]", m.pre_drv)]),
@@ -894,6 +900,8 @@ def build(out_path, info_path=None, dev_path=None):
              [(f"Y, {i}", J_a[i]) for i in range(NEQ)], 128, 500, 32),
             ("ad_jacd_vjp_st", "AD_STAGE_IN(y), AD_STAGE_IN(M), AD_STAGE_IN(W) AD_STAGE_PARAMS",
              [(f"Y, {i}", D_a[i]) for i in range(NEQ)] +
+             [(f"MB, {r}", D_a[NEQ + r]) for r in range(R)], 128, 500, 32),
+            ("ad_jacd_mbar_st", "AD_STAGE_IN(y), AD_STAGE_IN(W) AD_STAGE_PARAMS",
              [(f"MB, {r}", D_a[NEQ + r]) for r in range(R)], 128, 500, 32)]:
         text, st = emit_staged_fn(d, name, params, outs, regs, prefetch=pf, extra=extra)
         staged.append(text)

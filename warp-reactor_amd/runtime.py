@@ -323,12 +323,15 @@ class Context:
     def set_coop(self, cells, waves=None):
         """Advance the `cells` most expensive cells of each step with whole
         waves (wave 0 of the first `waves` blocks; default waves = cells);
-        0 turns it off.  Results are bit-identical."""
-        waves = cells if waves is None else waves
+        0 turns it off, -1 (the context's default) takes one cell per 2,048
+        cells, at most 128.  Results are bit-identical."""
+        if waves is None:
+            waves = cells if cells > 0 else 0
         _check(self._lib.wr_set_coop(self._h, int(cells), int(waves)), "wr_set_coop")
 
     def coop(self):
-        """(cells, waves, LDS bytes per cooperative wave)."""
+        """(cells, waves, LDS bytes per cooperative wave) of the current
+        setting for this context's grid."""
         v = [ctypes.c_int() for _ in range(3)]
         _check(self._lib.wr_get_coop(self._h, *[ctypes.byref(x) for x in v]), "wr_get_coop")
         return tuple(x.value for x in v)
