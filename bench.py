@@ -232,6 +232,8 @@ def main():
                     help="skip the 128^3 log-normal recipe measurement")
     ap.add_argument("--no-legacy", action="store_true",
                     help="skip the legacy-local policy run")
+    ap.add_argument("--no-cta32", action="store_true",
+                    help="skip the gridwide-cta32 policy run")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (32^3, 20 outer steps)")
     ap.add_argument("--no-graph", action="store_true",
@@ -381,6 +383,8 @@ def main():
                                                    ages)
     if not args.no_legacy and args.advance_kernel == 1:
         line["legacy_local"] = legacy_measure(ctx, init, args.max_outer)
+    if not args.no_cta32 and args.advance_kernel == 1:
+        line["gridwide_cta32"] = cta32_measure(ctx, init, args.max_outer, cells, n, wr)
     if not args.no_lognormal and args.advance_kernel == 1 and args.grid == 64:
         line["lognormal128"] = lognormal128_measure(wr, ctx_shape)
     if not args.no_adjoint:
@@ -553,6 +557,40 @@ def legacy_measure(ctx, init, max_outer):
     return {"run_seconds": el, "value": info["cell_advances"] / el, "unit": "cell-advances/s",
             "outer_steps": info["completed_global_steps"], "cell_advances": info["cell_advances"],
             "status": info["status"], "stop_reason": info["stop_reason"]}
+
+
+def cta32_measure(ctx, init, max_outer, ref_cells, n, wr):
+    """The Mojo front end's gridwide-cta32 policy (one step controller per
+    32-cell tile) on the same cells: one run, timed, and its final states
+    against the gridwide-independent run's (the reference design's acceptance
+    rule for the policy: the comparator's tolerances)."""
+    ctx.set_policy("gridwide-cta32")
+    try:
+        ctx.upload(init)
+        t0 = time.perf_counter()
+        info = ctx.run(perturb=True, max_steps=max_outer)
+        el = time.perf_counter() - t0
+        cells = np.zeros(n, dtype=wr.CELL_DTYPE)
+        ctx.download(cells)
+    finally:
+        ctx.set_policy("gridwide")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "independent.bin")
+        grid = round(n ** (1.0 / 3.0))
+        wr.write_final_states(ref_cells, grid, path)
+        cmp = wr.compare_final_states(cells, grid, path)
+    st = cells["stats"]
+    print(f"# gridwide-cta32: {el:.3f} s, {info['completed_global_steps']} steps, comparator "
+          f"{'PASS' if cmp['pass'] else 'FAIL'}", file=sys.stderr, flush=True)
+    return {"run_seconds": el, "value": info["cell_advances"] / el, "unit": "cell-advances/s",
+            "outer_steps": info["completed_global_steps"], "cell_advances": info["cell_advances"],
+            "status": info["status"], "limiting_cell": info["limiting_cell"],
+            "attempts": int(st["internal_steps"].sum()),
+            "attempts_independent": int(ref_cells["stats"]["internal_steps"].sum()),
+            "accepted_steps": int(st["accepted_steps"].sum()),
+            "vs_independent_comparator": {k: cmp[k] for k in (
+                "pass", "failures", "max_non_deuterium_species_rel_error",
+                "max_thermodynamic_rel_error", "max_deuterium_species_rel_error")}}
 
 
 def adjoint_measure(wr, grid=32, nsteps=20, seed=2, directions=8, delta=1e-6):

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WR_ABI_VERSION 1
+#define WR_ABI_VERSION 2
 
 #define WR_NUM_SPEC 14
 #define WR_NEQS 15
@@ -113,6 +113,8 @@ typedef struct wr_run_info {
     int32_t failed_cell;        /* cell whose failure ended the run, or -1 */
     double grid_time;           /* physical time of the grid after the run */
     uint64_t cell_advances;     /* sum over steps of integrated cells */
+    int32_t limiting_cell;      /* (dt, lowest cell) holder of the last planned step, or -1 */
+    int32_t reserved_;
 } wr_run_info;
 
 typedef struct wr_context wr_context;
@@ -132,6 +134,9 @@ int wr_run_cells(wr_collapse_state *cells, int num_cells, int perturb,
 /* ---- context ---------------------------------------------------------------- */
 int wr_context_create(int device, int capacity_cells, wr_context **out);
 void wr_context_destroy(wr_context *ctx);
+/* cells: host memory, or device memory of the context's device (the copy
+ * direction follows from unified addressing; device data must be complete
+ * before the call, and the call returns after its copy has finished). */
 int wr_upload_cells(wr_context *ctx, const wr_collapse_state *cells, int num_cells);
 int wr_download_cells(wr_context *ctx, wr_collapse_state *cells, int num_cells);
 
@@ -197,8 +202,19 @@ int wr_advance(wr_context *ctx, int completed_global_steps, double next_grid_tim
  * cell loop stops at the lowest failing cell, the cells after it keep their
  * state of that step's start.  grid_time is not used (each cell keeps its own
  * time); wr_dt_history records no steps.  The legacy-local policy runs on the
- * queue-fed kernel in FP64 (FP32 factorisation: WR_ERR_ARGUMENT). */
-enum wr_policy { WR_POLICY_GRIDWIDE = 0, WR_POLICY_LEGACY_LOCAL = 1 };
+ * queue-fed kernel in FP64 (FP32 factorisation: WR_ERR_ARGUMENT).
+ * WR_POLICY_GRIDWIDE_CTA32: the gridwide-cta32 policy of the reference's Mojo
+ * front end (reproducer.mojo:2829-3093, driver 3604-3621; --driver-policy
+ * gridwide-cta32): the gridwide outer step with the density updates evaluated
+ * as (dt * driver) / tff, and the cells advanced in tiles of 32 consecutive
+ * cells that share one ROS2S step controller (tile error = the largest cell
+ * error, a singular factorisation in any cell halves h for the tile, an
+ * accepted final trial ends exactly at dt_grid; the Jacobian counter counts
+ * every trial).  A failing tile ends the run with the code of the lowest
+ * failing tile (failed_cell = its first cell); the Mojo driver then raises,
+ * so the cells' states after a failed step are not defined.  FP64 reference
+ * integrator only (FP32 factorisation, Jacobian reuse: WR_ERR_ARGUMENT). */
+enum wr_policy { WR_POLICY_GRIDWIDE = 0, WR_POLICY_LEGACY_LOCAL = 1, WR_POLICY_GRIDWIDE_CTA32 = 2 };
 int wr_set_policy(wr_context *ctx, int policy);
 
 /* Advance kernel of subsequent steps: 1 (default) = queue-fed persistent
@@ -294,8 +310,10 @@ int wr_last_advance_timing(const wr_context *ctx, double *total_ms, int *launche
  * code generator from the reference's rate code; wr_adjoint_ntheta() of them):
  * M_r scales reaction r in the species rates, in the energy rate where the
  * network books its heat, and in the Jacobian.
- * States are [18][n] host arrays, field-major: xn[0..13], T, e, rho,
- * density_driver.  e of the first state does not enter the map (eos_rt
+ * States are [18][n] arrays, field-major: xn[0..13], T, e, rho,
+ * density_driver; states, seeds and gradients may be host memory or device
+ * memory of the context's device (as for wr_upload_cells), multipliers are
+ * host memory.  e of the first state does not enter the map (eos_rt
  * recomputes it from T).  Perturbations are not part of the map. */
 int wr_adjoint_ntheta(void);
 /* Run `nsteps` outer steps of the uploaded cells with multipliers M (no

@@ -26,6 +26,7 @@
 #include <math.h>
 #include <float.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define NSPEC 14
@@ -980,6 +981,272 @@ int or_run_legacy(or_cell *cells, int n, int perturb, int max_steps, int *comple
     }
     *completed_out = completed;
     return failure;
+}
+
+/* ---- gridwide-cta32 policy of the reference's Mojo front end ---------------
+ * reproducer.mojo:2829-3093 with the driver loop of main (3604-3621).  The
+ * outer step is the gridwide-independent one (prepare, grid minimum of
+ * (dt, cell), density update, advance, commit), with these differences:
+ *  - both density-driver updates evaluate (dt * driver) / tff
+ *    (reproducer.mojo:2858-2860, 2891-2893), where reproducer.cpp:7730 and
+ *    7759 evaluate dt * (driver / tff);
+ *  - cells advance in tiles of `tile` consecutive cells (TileCells = 32,
+ *    reproducer.mojo:40), one ROS2S step controller per tile
+ *    (integrate_gridwide_tile, 2910-3048): each trial runs on the tile's
+ *    participating cells in index order; a singular factorisation stops the
+ *    trial there and halves h for the tile; otherwise the tile error is the
+ *    largest cell error (DBL_MAX if any is not finite) and the Gustafsson
+ *    controller acts on it; an accepted final trial sets x = dt_grid exactly;
+ *  - the Jacobian is evaluated, and counted, on every trial (ros2s_trial,
+ *    2460-2560: eval_jacobian 2420-2424 before the decomposition);
+ *  - the first failing tile in index order ends the step and the run
+ *    (advance_gridwide_step 3079-3088; main raises, so no state is written).
+ * With tile = 1 and cpp_order = 1 (the density updates evaluated in the
+ * order of reproducer.cpp) each cell has its own controller, and the run
+ * equals or_run_gridwide in every state bit and every counter except
+ * jacobian_calls (one per trial here, one per accepted step there). */
+
+/* ros2s_trial, reproducer.mojo:2460-2560: one trial of step h from the cell's
+ * (xn, e); ynew and the error norm out, the cell unchanged. */
+static int cta_trial(const or_burn_t *s, double h, double *ynew, double *err_out, or_stats *st) {
+    double y[NEQ], fjac[NEQ * NEQ], e[NEQ * NEQ], ak1[NEQ], ak2[NEQ], work[NEQ], dy[NEQ];
+    int ip[NEQ];
+    for (int n = 0; n < NSPEC; ++n) y[n] = s->xn[n];
+    y[NSPEC] = s->e;
+    hook_jac(y, fjac);
+    st->jacobian_calls += 1;
+    const double fac = 1.0 / (h * c_gamma);
+    for (int i = 0; i < NEQ; ++i) {
+        for (int j = 0; j < NEQ; ++j) e[i * NEQ + j] = -fjac[i * NEQ + j];
+        e[i * NEQ + i] += fac;
+    }
+    if (or_lu(e, ip) != 0) return OR_LU_ERROR;
+    st->decompositions += 1;
+    hook_rhs(y, ak1);
+    or_lu_solve(e, ip, ak1);
+    st->linear_solves += 1;
+    for (int i = 0; i < NEQ; ++i) {
+        ynew[i] = y[i] + c_a21 * ak1[i];
+        ak2[i] = (c_c21 / h) * ak1[i];
+    }
+    hook_rhs(ynew, dy);
+    for (int i = 0; i < NEQ; ++i) ak2[i] += dy[i];
+    or_lu_solve(e, ip, ak2);
+    st->linear_solves += 1;
+    for (int i = 0; i < NEQ; ++i) {
+        ynew[i] = y[i] + c_a31 * ak1[i] + c_a32 * ak2[i];
+        work[i] = (c_c31 * ak1[i] + c_c32 * ak2[i]) / h;
+    }
+    hook_rhs(ynew, dy);
+    for (int i = 0; i < NEQ; ++i) work[i] += dy[i];
+    or_lu_solve(e, ip, work);
+    st->linear_solves += 1;
+    for (int i = 0; i < NEQ; ++i) {
+        const double ak3 = work[i];
+        ynew[i] = y[i] + c_b1 * ak1[i] + c_b2 * ak2[i] + c_b3 * ak3;
+        work[i] = c_e1 * ak1[i] + c_e2 * ak2[i] + c_e3 * ak3;
+    }
+    st->rhs_calls += 3;
+    st->internal_steps += 1;
+    /* error_norm, reproducer.mojo:2448-2457 */
+    double err = 0.0;
+    for (int i = 0; i < NEQ; ++i) {
+        const double sk = atol_of(i) + rtol_of(i) * ref_max(fabs(y[i]), fabs(ynew[i]));
+        const double term = work[i] / sk;
+        err += term * term;
+    }
+    *err_out = sqrt(err / (double)NEQ);
+    return OR_SUCCESS;
+}
+
+/* integrate_gridwide_tile, reproducer.mojo:2910-3048: cells [b, e) of which
+ * those with act[c] participate; returns OR_SUCCESS or the tile's failure. */
+static int cta_tile(or_cell *cells, const unsigned char *act, int b, int e, double dt_grid,
+                    double next_grid_time, int completed) {
+    double cand[64][NEQ];
+    int participants = 0;
+    for (int c = b; c < e; ++c) participants += act[c] ? 1 : 0;
+    if (participants == 0) return OR_SUCCESS;
+    double h = dt_grid, x = 0.0, hacc = 0.0, erracc = 1.0;
+    int reject = 0, nsing = 0, n_step = 0, n_accept = 0;
+    while (x < dt_grid) {
+        if (n_step > g_integ_max_steps) return OR_TOO_MANY_STEPS;
+        if (0.1 * fabs(h) <= fabs(x) * k_uround) return OR_DT_UNDERFLOW;
+        int final_trial = 0;
+        if (x + h * 1.0001 >= dt_grid) {
+            h = dt_grid - x;
+            final_trial = 1;
+        }
+        int singular = 0, nonfinite = 0;
+        double err_tile = 0.0;
+        for (int c = b; c < e; ++c) {
+            if (!act[c] || singular) continue;
+            double err = 0.0;
+            if (cta_trial(&cells[c].current, h, cand[c - b], &err, &cells[c].stats) != OR_SUCCESS)
+                singular = 1;
+            else if (!isfinite(err))
+                nonfinite = 1;
+            else
+                err_tile = ref_max(err_tile, err);
+        }
+        if (singular) {
+            nsing += 1;
+            if (nsing >= 5) return OR_LU_ERROR;
+            h *= 0.5;
+            reject = 1;
+            continue;
+        }
+        if (nonfinite) err_tile = DBL_MAX;
+        n_step += 1;
+        const double raw_fac = OR_CBRT(err_tile) / k_safe;
+        const double fac_step = ref_max(1.0 / k_fac_max, ref_min(1.0 / k_fac_min, raw_fac));
+        double hnew = h / fac_step;
+        if (err_tile <= 1.0) {
+            n_accept += 1;
+            if (n_accept > 1) {
+                const double facgus = ref_max(
+                    1.0 / k_fac_max,
+                    ref_min(1.0 / k_fac_min,
+                            (hacc / h) * OR_CBRT((err_tile * err_tile) / erracc) / k_safe));
+                hnew = h / ref_max(fac_step, facgus);
+            }
+            hacc = h;
+            erracc = ref_max(1.0e-2, err_tile);
+            for (int c = b; c < e; ++c) {
+                if (!act[c]) continue;
+                for (int n = 0; n < NSPEC; ++n) cells[c].current.xn[n] = cand[c - b][n];
+                cells[c].current.e = cand[c - b][NSPEC];
+                cells[c].stats.accepted_steps += 1;
+            }
+            x = final_trial ? dt_grid : x + h;
+            if (fabs(hnew) > dt_grid) hnew = dt_grid;
+            if (reject) hnew = ref_min(fabs(hnew), fabs(h));
+            reject = 0;
+            h = hnew;
+            continue;
+        }
+        reject = 1;
+        h = hnew;
+        if (n_accept >= 1)
+            for (int c = b; c < e; ++c)
+                if (act[c]) cells[c].stats.rejected_steps += 1;
+    }
+    for (int c = b; c < e; ++c) {
+        if (!act[c]) continue;
+        or_post_step((double *)&cells[c].current);
+        cells[c].time = next_grid_time;
+        cells[c].completed_steps = completed + 1;
+    }
+    return OR_SUCCESS;
+}
+
+/* The driver loop (reproducer.mojo:3604-3621) over advance_gridwide_step
+ * (3051-3093): prepare_grid_timestep (2829-2871), setup_gridwide_cells
+ * (2874-2907), the tiles in index order.  tile in [1, 64]; max_steps <= 0
+ * means 1000; cpp_order: evaluate the density updates as reproducer.cpp
+ * does (a pinning aid; the policy is cpp_order = 0).  *limiting_cell: the last step's (dt, lowest cell) holder. */
+int or_run_cta(or_cell *cells, int n, int perturb, int max_steps, int tile, int cpp_order,
+               int *completed_out, int *limiting_cell) {
+    const int limit = max_steps > 0 ? max_steps : 1000;
+    if (tile < 1 || tile > 64) return OR_BAD_INPUTS;
+    unsigned char *act = (unsigned char *)calloc(n > 0 ? (size_t)n : 1, 1);
+    int *rc = (int *)calloc(n / tile + 1, sizeof(int));
+    double grid_time = 0.0;
+    int completed = *completed_out, result = OR_SUCCESS;
+    *limiting_cell = -1;
+    for (int step = 0; step < limit; ++step) {
+        /* prepare_grid_timestep: the lowest failing cell's code wins (the
+         * loop returns at the first) */
+        double dt_grid = DBL_MAX;
+        int cell_min = -1, active_count = 0, failure = OR_SUCCESS;
+        for (int c = 0; c < n && failure == OR_SUCCESS; ++c) {
+            or_cell *s = &cells[c];
+            act[c] = 0;
+            if (s->completed_steps < completed) continue;
+            if (s->completed_steps != completed || s->time != grid_time) {
+                failure = OR_BAD_INPUTS;
+                break;
+            }
+            or_apply_perturbation(s, c, step, perturb);
+            const double dt = or_collapse_timestep(s);
+            if (dt <= 0.0 || !valid_positive(s->density_driver)) {
+                failure = OR_BAD_INPUTS;
+                break;
+            }
+            if (dt < 10.0) {
+                const double tff = dt / 1.0e-1;
+                if (cpp_order)
+                    s->density_driver += dt * (s->density_driver / tff);
+                else
+                    s->density_driver += dt * s->density_driver / tff;
+                continue;
+            }
+            act[c] = 1;
+            active_count += 1;
+            if (dt < dt_grid || (dt == dt_grid && c < cell_min)) {
+                dt_grid = dt;
+                cell_min = c;
+            }
+        }
+        *limiting_cell = cell_min;
+        if (failure != OR_SUCCESS) {
+            result = failure;
+            break;
+        }
+        if (active_count == 0) break;
+        const double next = grid_time + dt_grid;
+        if (!valid_positive(dt_grid) || !isfinite(next)) {
+            result = OR_BAD_INPUTS;
+            break;
+        }
+        /* setup_gridwide_cells */
+        int participants = 0;
+        for (int c = 0; c < n; ++c) {
+            if (!act[c]) continue;
+            or_cell *s = &cells[c];
+            const double local_dt = or_collapse_timestep(s);
+            if (local_dt < 10.0) {
+                act[c] = 0;
+                continue;
+            }
+            const double tff = local_dt / 1.0e-1;
+            const double old_density = s->density_driver;
+            if (cpp_order)
+                s->density_driver += dt_grid * (s->density_driver / tff);
+            else
+                s->density_driver += dt_grid * s->density_driver / tff;
+            if (!valid_positive(s->density_driver) || s->density_driver > 2.0e-6) {
+                act[c] = 0;
+                continue;
+            }
+            const double ratio = s->density_driver / old_density;
+            for (int k = 0; k < NSPEC; ++k) s->current.xn[k] *= ratio;
+            s->current.rho *= ratio;
+            eos_rt(&s->current);
+            participants += 1;
+        }
+        if (participants == 0) break;
+        /* tiles are independent until the first failure: run them in
+         * parallel, then take the lowest failing tile's code */
+        const int ntiles = (n + tile - 1) / tile;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int t = 0; t < ntiles; ++t) {
+            const int b = t * tile, e = (b + tile < n) ? b + tile : n;
+            rc[t] = cta_tile(cells, act, b, e, dt_grid, next, completed);
+        }
+        for (int t = 0; t < ntiles && failure == OR_SUCCESS; ++t)
+            if (rc[t] != OR_SUCCESS) failure = rc[t];
+        if (failure != OR_SUCCESS) {
+            result = failure;
+            break;
+        }
+        grid_time = next;
+        completed += 1;
+    }
+    free(act);
+    free(rc);
+    *completed_out = completed;
+    return result;
 }
 
 int or_sizeof_cell(void) { return (int)sizeof(or_cell); }

@@ -48,6 +48,8 @@ class _Cpu:
                         "set_lu_precision": (None, [_I, _I]),
                         "set_jacobian_reuse": (None, [_I]),
                         "run_legacy": (_I, [_VP, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+                        "run_cta": (_I, [_VP, _I, _I, _I, _I, _I, ctypes.POINTER(_I),
+                                         ctypes.POINTER(_I)]),
                         "mixed_fallbacks": (None, [_VP, _VP]),
                         "reset_mixed_fallbacks": (None, []),
                         "lu32": (_I, [_VP, _VP]), "lu_solve32": (None, [_VP, _VP, _VP]),
@@ -217,6 +219,17 @@ class _Cpu:
         r = self._run_gridwide(_p(cells), len(cells), int(perturb), max_steps, ctypes.byref(done))
         cells["reserved_"] = 0
         return r, done.value
+
+    def run_cta(self, cells, perturb=True, max_steps=0, tile=32, completed0=0, cpp_order=False):
+        """The Mojo front end's gridwide-cta32 policy (reproducer.mojo:2829-3093,
+        3604-3621) with `tile` cells per step controller: (status, completed
+        steps, limiting cell of the last step or -1).  cpp_order: the density
+        updates in reproducer.cpp's operation order (for pinning)."""
+        done, lim = _I(completed0), _I(-1)
+        r = self._run_cta(_p(cells), len(cells), int(perturb), max_steps, tile, int(cpp_order),
+                          ctypes.byref(done), ctypes.byref(lim))
+        cells["reserved_"] = 0
+        return r, done.value, lim.value
 
 
 class Oracle(_Cpu):

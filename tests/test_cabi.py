@@ -22,7 +22,7 @@ def test_library_exports_every_declared_symbol(wr):
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(wr.runtime.EXPORTED)
-    assert lib.wr_abi_version() == 1
+    assert lib.wr_abi_version() == 2
 
 
 def test_initial_state_matches_oracle(wr, oracle):

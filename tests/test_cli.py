@@ -36,8 +36,9 @@ def test_bad_arguments_fail_with_usage(args, tmp_path):
 
 
 def test_driver_policy_option(tmp_path):
-    assert "--driver-policy gridwide-independent|legacy-local" in run("--help").stderr
-    for args in (["--driver-policy"], ["--driver-policy", "gridwide-cta32"]):
+    assert "--driver-policy gridwide-independent|legacy-local|gridwide-cta32" in \
+        run("--help").stderr
+    for args in (["--driver-policy"], ["--driver-policy", "gridwide-cta64"]):
         r = run(*args, cwd=tmp_path)
         assert r.returncode == 1 and "usage: " in r.stderr and r.stdout == ""
     assert not os.listdir(tmp_path)

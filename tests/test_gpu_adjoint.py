@@ -148,6 +148,33 @@ def test_autograd_function_gradients(wr, adj):
     assert np.array_equal(_bits(omb.sum(axis=1)), _bits(gth))
 
 
+def test_autograd_device_tensors_match_host_tensors(wr, adj):
+    """theta and s0 on cuda:0: the cells, seeds and gradients cross the C ABI
+    as device addresses and the results stay on the GPU; final states equal
+    the host path's bit for bit, the gradients to 1e-12 (torch forms the loss
+    seed and the theta gradient's sum over cells with its own GPU kernels)."""
+    import importlib
+    import torch
+    ag = importlib.import_module("warp-reactor_amd.autograd")
+    n = 4 ** 3
+    cells, _ = _cells(wr, adj, n)
+    base = ag.state_of(cells)
+    out = {}
+    for dev in ("cpu", "cuda:0"):
+        s0 = torch.tensor(base, dtype=torch.float64, device=dev, requires_grad=True)
+        theta = torch.zeros(wr.adjoint_ntheta(), dtype=torch.float64, device=dev,
+                            requires_grad=True)
+        with wr.Context(n) as ctx:
+            sN = ag.collapse(theta, s0, ctx, nsteps=NSTEPS)
+            assert sN.device == s0.device
+            ag.h2_log_t_loss(sN).backward()
+        assert theta.grad.device == theta.device and s0.grad.device == s0.device
+        out[dev] = (sN.detach().cpu().numpy(), s0.grad.cpu().numpy(), theta.grad.cpu().numpy())
+    assert np.array_equal(_bits(out["cpu"][0]), _bits(out["cuda:0"][0]))
+    np.testing.assert_allclose(out["cuda:0"][1], out["cpu"][1], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(out["cuda:0"][2], out["cpu"][2], rtol=1e-12, atol=0)
+
+
 def test_autograd_backward_refuses_a_replaced_tape(wr, adj):
     """Two forwards on one context, then the first one's backward: the
     context's tape belongs to the second recording, so the first backward

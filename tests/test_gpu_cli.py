@@ -120,6 +120,28 @@ def test_cli_legacy_local_policy(tmp_path, wr, oracle_wr):
     assert data == wr.pack_final_states(ocells, 2).tobytes()
 
 
+def test_cli_gridwide_cta32_policy(tmp_path, wr, oracle_wr):
+    """--driver-policy gridwide-cta32 (the Mojo front end's policy): its
+    summary lines -- the policy, the oracle's step count and the limiting
+    cell, printed as reproducer.mojo:3655-3664 does -- the file bytes of the
+    oracle's grid, and the Mojo manifest's tile size and policy tag."""
+    ocells = oracle_wr.make_cells(27, wr.CELL_DTYPE)
+    orr, odone, olim = oracle_wr.run_cta(ocells, True, 1000, tile=32)
+    assert orr == 1
+    out = tmp_path / "cta.bin"
+    r = run(["--grid", "3", "--driver-policy", "gridwide-cta32", "--no-compare-final-state",
+             "--write-final-state", str(out)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    lim = olim if olim >= 0 else 2 ** 63 - 1
+    assert lines[3:6] == ["driver policy: gridwide-cta32",
+                          f"completed global collapse steps: {odone}", f"limiting cell: {lim}"]
+    data = (tmp_path / "final_states_grid3_hip_cta32.bin").read_bytes()
+    assert data == wr.pack_final_states(ocells, 3).tobytes() == out.read_bytes()
+    manifest = dict(l.split("=", 1) for l in (tmp_path / "cta.bin.manifest").read_text().splitlines())
+    assert manifest["policy"] == "gridwide-cta32-v1" and manifest["tile_cells"] == "32"
+
+
 def test_write_final_state_file_and_manifest(tmp_path):
     """--write-final-state FILE (the Mojo front end's option,
     reproducer.mojo:3318-3340, 3529-3535): the packed records go to FILE --

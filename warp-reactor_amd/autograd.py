@@ -16,7 +16,10 @@ that run with its controller decisions frozen -- dt_grid of each step and the
 accepted step sizes -- recomputing each step from its checkpoint and the
 substeps inside it.  e of s0 does not enter the map (the advance recomputes it
 from T), so its gradient is zero.  PyTorch is only the binding: the compute
-runs in the HIP library; tensors cross the boundary as float64 host arrays.
+runs in the HIP library.  Tensors on the GPU stay there: the cells, seeds and
+gradients cross the boundary as device addresses (the library copies them on
+its own stream after torch's work on them is complete); CPU tensors cross as
+float64 host arrays.
 """
 from __future__ import annotations
 
@@ -41,6 +44,30 @@ def cells_from_state(s0):
     return cells
 
 
+# a cell record (216 bytes) as 27 float64 slots: rho, T, e, xn[0..13], time,
+# density_driver, (completed_steps, reserved) = 0, seven counters = 0
+_SLOTS = 27
+_XN = slice(3, 17)
+
+
+def cells_tensor_from_state(s0):
+    """Cell records at completed step 0 and time 0, as an (n, 27) float64
+    tensor on s0's device, from an (18, n) state tensor."""
+    n = s0.shape[1]
+    c = torch.zeros((n, _SLOTS), dtype=torch.float64, device=s0.device)
+    c[:, _XN] = s0[:14].T
+    c[:, 1] = s0[14]
+    c[:, 2] = s0[15]
+    c[:, 0] = s0[16]
+    c[:, 18] = s0[17]
+    return c
+
+
+def state_of_cells_tensor(c):
+    """(18, n) state tensor of an (n, 27) cell-record tensor."""
+    return torch.cat([c[:, _XN].T, c[:, 1][None], c[:, 2][None], c[:, 0][None], c[:, 18][None]])
+
+
 def state_of(cells):
     """(18, n) state array of a cell array."""
     c = cells
@@ -56,25 +83,44 @@ class CollapseAdjoint(torch.autograd.Function):
         if theta.shape != (adjoint_ntheta(),):
             raise ValueError(f"theta must have shape ({adjoint_ntheta()},)")
         M = torch.exp(theta.detach().to(torch.float64)).cpu().numpy()
-        cells = cells_from_state(s0.detach().to(torch.float64).cpu().numpy())
+        fctx.context = context
+        fctx.M = M
+        fctx.theta_device = theta.device
+        if s0.is_cuda:
+            cells = cells_tensor_from_state(s0.detach().to(torch.float64)).contiguous()
+            torch.cuda.synchronize(s0.device)  # the records are complete before the copy
+            context.upload_ptr(cells.data_ptr(), cells.shape[0])
+            context.adjoint_record(nsteps, M, hcap)
+            context.download_ptr(cells.data_ptr(), cells.shape[0])
+            fctx.generation = context.adjoint_generation()  # the tape this forward recorded
+            return state_of_cells_tensor(cells)
+        cells = cells_from_state(s0.detach().to(torch.float64).numpy())
         context.upload(cells)
         context.adjoint_record(nsteps, M, hcap)
         context.download(cells)
-        fctx.context = context
-        fctx.generation = context.adjoint_generation()  # the tape this forward recorded
-        fctx.M = M
-        return torch.from_numpy(state_of(cells)).to(device=s0.device)
+        fctx.generation = context.adjoint_generation()
+        return torch.from_numpy(state_of(cells))
 
     @staticmethod
     def backward(fctx, gN):
         # fails (WarpReactorError) if a later forward on the same context
         # replaced this forward's tape
-        g0, mbar = fctx.context.adjoint_backward(gN.detach().to(torch.float64).cpu().numpy(),
+        if gN.is_cuda:
+            g = gN.detach().to(torch.float64).contiguous()
+            n = g.shape[1]
+            g0 = torch.empty((FIELDS, n), dtype=torch.float64, device=g.device)
+            mbar = torch.empty((adjoint_ntheta(), n), dtype=torch.float64, device=g.device)
+            torch.cuda.synchronize(g.device)
+            fctx.context.adjoint_backward_ptr(g.data_ptr(), g0.data_ptr(), mbar.data_ptr(),
+                                              fctx.generation)
+            M = torch.from_numpy(fctx.M).to(device=g.device)
+            gtheta = mbar.sum(dim=1) * M  # dM/dtheta = M
+            return gtheta.to(fctx.theta_device), g0, None, None, None
+        g0, mbar = fctx.context.adjoint_backward(gN.detach().to(torch.float64).numpy(),
                                                  generation=fctx.generation)
         gtheta = mbar.sum(axis=1) * fctx.M  # dM/dtheta = M
-        dev = gN.device
-        return (torch.from_numpy(gtheta).to(device=dev), torch.from_numpy(g0).to(device=dev),
-                None, None, None)
+        return (torch.from_numpy(gtheta).to(fctx.theta_device), torch.from_numpy(g0), None, None,
+                None)
 
 
 def collapse(theta, s0, context, nsteps=20, hcap=512):

@@ -1,16 +1,19 @@
 // wr_reproducer: command-line front end with the reference reproducer's
 // interface and output (reproducer.cpp:8201-8277, 8591-8703) on the gfx950
 // library.  The driver policy is the reference's HIP build's
-// (gridwide-independent) unless --driver-policy legacy-local asks for its CPU
-// build's (each cell steps with its own free-fall dt, 8628-8648).
+// (gridwide-independent) unless --driver-policy asks for its CPU build's
+// (legacy-local: each cell steps with its own free-fall dt, 8628-8648) or for
+// the Mojo front end's gridwide-cta32 (one step controller per 32-cell tile,
+// reproducer.mojo:2829-3093; prints the limiting cell as that front end does,
+// 3663-3664).
 //
 //   wr_reproducer [--grid N] [--perturb|--no-perturb]
 //                 [--compare-final-state FILE|--no-compare-final-state]
-//                 [--driver-policy gridwide-independent|legacy-local]
+//                 [--driver-policy gridwide-independent|legacy-local|gridwide-cta32]
 //                 [--write-final-state FILE]
 //
-// Writes final_states_grid<N>_hip.bin (_hip_legacy_local.bin for the
-// legacy-local policy; 172-byte records) for N > 1, prints the final state
+// Writes final_states_grid<N>_hip.bin (_hip_legacy_local.bin, _hip_cta32.bin
+// for the other policies; 172-byte records) for N > 1, prints the final state
 // for N = 1, exit status 0 on success and 1 on failure.  --write-final-state
 // FILE (the option of the reference's Mojo front end, reproducer.mojo:
 // 3283-3340, 3529-3535) also writes the packed records to FILE and a sidecar
@@ -39,7 +42,7 @@ struct Options {
     int grid_dim = 64;
     bool perturb = true;
     bool show_help = false;
-    bool legacy_local = false;
+    int policy = WR_POLICY_GRIDWIDE;
     std::string compare_path;
     std::string write_path;
 };
@@ -52,9 +55,17 @@ void print_usage(const char* program) {
     fprintf(stderr,
             "usage: %s [--grid N] [--perturb|--no-perturb]"
             " [--compare-final-state FILE|--no-compare-final-state]"
-            " [--driver-policy gridwide-independent|legacy-local]"
+            " [--driver-policy gridwide-independent|legacy-local|gridwide-cta32]"
             " [--write-final-state FILE]\n",
             program);
+}
+
+// Policy names: as --driver-policy spells them, or as the manifest tags them
+// (the Mojo front end tags its own policy gridwide-cta32-v1, reproducer.mojo:3299).
+const char* policy_name(int policy, bool manifest) {
+    if (policy == WR_POLICY_LEGACY_LOCAL) return "legacy-local";
+    if (policy == WR_POLICY_GRIDWIDE_CTA32) return manifest ? "gridwide-cta32-v1" : "gridwide-cta32";
+    return "gridwide-independent";
 }
 
 bool file_exists(const std::string& path) {
@@ -68,7 +79,7 @@ bool file_exists(const std::string& path) {
 // packed records in cell order, never over an existing file, and a sidecar
 // manifest of key=value lines.
 bool write_final_state_file(const std::vector<wr_collapse_state>& cells, int grid_dim,
-                            const std::string& path, bool legacy_local) {
+                            const std::string& path, int policy) {
     const std::string manifest = path + ".manifest";
     if (file_exists(path) || file_exists(manifest)) {
         fprintf(stderr, "refusing to overwrite final-state output: %s\n", path.c_str());
@@ -92,10 +103,11 @@ bool write_final_state_file(const std::vector<wr_collapse_state>& cells, int gri
         return false;
     }
     fprintf(m,
-            "format=packed-final-state-v1\ncell_count=%d\ngrid_dim=%d\npolicy=%s\n"
-            "network_source_sha256=%s\nbackend=%s-gfx950\nabi=%d\n",
-            n, grid_dim, legacy_local ? "legacy-local" : "gridwide-independent",
-            WR_NETWORK_SHA256, kBackend, wr_abi_version());
+            "format=packed-final-state-v1\ncell_count=%d\ngrid_dim=%d\n", n, grid_dim);
+    // the Mojo manifest's tile size and policy tag for its own policy (3293-3307)
+    if (policy == WR_POLICY_GRIDWIDE_CTA32) fprintf(m, "tile_cells=32\n");
+    fprintf(m, "policy=%s\nnetwork_source_sha256=%s\nbackend=%s-gfx950\nabi=%d\n",
+            policy_name(policy, true), WR_NETWORK_SHA256, kBackend, wr_abi_version());
     if (fclose(m) != 0) return false;
     printf("final-state output: %s (%d packed records, manifest %s)\n", path.c_str(), n,
            manifest.c_str());
@@ -144,11 +156,16 @@ bool parse_args(int argc, char** argv, Options& o) {
         }
         if (arg == "--driver-policy") {
             const std::string v = i + 1 < argc ? argv[++i] : "";
-            if (v != "gridwide-independent" && v != "legacy-local") {
+            if (v == "gridwide-independent") {
+                o.policy = WR_POLICY_GRIDWIDE;
+            } else if (v == "legacy-local") {
+                o.policy = WR_POLICY_LEGACY_LOCAL;
+            } else if (v == "gridwide-cta32") {
+                o.policy = WR_POLICY_GRIDWIDE_CTA32;
+            } else {
                 print_usage(argv[0]);
                 return false;
             }
-            o.legacy_local = v == "legacy-local";
             continue;
         }
         if (arg == "--write-final-state") {
@@ -213,14 +230,16 @@ CounterSummary summarize_counter(const std::vector<wr_collapse_state>& cells, si
     return s;
 }
 
-// The reference's CPU driver loop (reproducer.cpp:8628-8648) on the device:
-// context API with the legacy-local policy; HIP and API errors are reported
-// on stderr and fail the run.
-int run_legacy_local(std::vector<wr_collapse_state>& cells, bool perturb, int& completed) {
+// The reference's CPU driver loop (reproducer.cpp:8628-8648) or the Mojo
+// front end's gridwide-cta32 loop (reproducer.mojo:3604-3621) on the device:
+// context API with that policy; HIP and API errors are reported on stderr and
+// fail the run.  *limiting: the (dt, lowest cell) holder of the last prepare.
+int run_policy(std::vector<wr_collapse_state>& cells, bool perturb, int policy, int& completed,
+               int& limiting) {
     const int n = static_cast<int>(cells.size());
     wr_context* ctx = nullptr;
     int r = wr_context_create(0, n, &ctx);
-    if (r == WR_SUCCESS) r = wr_set_policy(ctx, WR_POLICY_LEGACY_LOCAL);
+    if (r == WR_SUCCESS) r = wr_set_policy(ctx, policy);
     if (r == WR_SUCCESS) r = wr_upload_cells(ctx, cells.data(), n);
     if (r == WR_SUCCESS) r = wr_run(ctx, perturb ? 1 : 0, 1000, 0, 0.0);
     wr_run_info info{};
@@ -229,6 +248,7 @@ int run_legacy_local(std::vector<wr_collapse_state>& cells, bool perturb, int& c
         if (d == WR_SUCCESS) d = wr_run_info_get(ctx, &info);
         if (d != WR_SUCCESS) r = d;
         completed = info.completed_global_steps;
+        limiting = info.limiting_cell;
     }
     if (r <= WR_ERR_HIP) fprintf(stderr, "%s\n", wr_last_error());
     wr_context_destroy(ctx);
@@ -260,11 +280,11 @@ int main(int argc, char** argv) {
     std::vector<wr_collapse_state> cells(static_cast<size_t>(num_cells));
     wr_make_collapse_states(cells.data(), num_cells);
 
-    int completed = 0;
+    int completed = 0, limiting = -1;
     const auto t0 = std::chrono::steady_clock::now();
-    const int result = o.legacy_local ? run_legacy_local(cells, o.perturb, completed)
-                                      : wr_run_cells(cells.data(), num_cells, o.perturb ? 1 : 0,
-                                                     &completed);
+    const int result = o.policy != WR_POLICY_GRIDWIDE
+                           ? run_policy(cells, o.perturb, o.policy, completed, limiting)
+                           : wr_run_cells(cells.data(), num_cells, o.perturb ? 1 : 0, &completed);
     const auto t1 = std::chrono::steady_clock::now();
     const double elapsed = std::chrono::duration<double>(t1 - t0).count();
     if (result != WR_SUCCESS) return finish(false, false);
@@ -284,8 +304,11 @@ int main(int argc, char** argv) {
     printf("Primordial chemistry collapse grid with ROS2S\n");
     printf("grid: %d^3 (%d cells)\n", o.grid_dim, num_cells);
     printf("perturbations: %s\n", o.perturb ? "enabled" : "disabled");
-    printf("driver policy: %s\n", o.legacy_local ? "legacy-local" : "gridwide-independent");
+    printf("driver policy: %s\n", policy_name(o.policy, false));
     printf("completed global collapse steps: %d\n", completed);
+    // no limiting cell: the Mojo front end prints its Int.MAX sentinel
+    if (o.policy == WR_POLICY_GRIDWIDE_CTA32)
+        printf("limiting cell: %lld\n", limiting >= 0 ? (long long)limiting : LLONG_MAX);
     if (num_cells == 1) {
         printf("representative cell completed steps: %d\n", rep.completed_steps);
         printf("representative cell physical time: %.17Le\n", static_cast<long double>(rep.time));
@@ -347,11 +370,14 @@ int main(int argc, char** argv) {
     }
 
     if (!o.write_path.empty() &&
-        !write_final_state_file(cells, o.grid_dim, o.write_path, o.legacy_local))
+        !write_final_state_file(cells, o.grid_dim, o.write_path, o.policy))
         return finish(false, comparison_done);
     if (o.grid_dim > 1) {
         const std::string file = "final_states_grid" + std::to_string(o.grid_dim) + "_" + kBackend +
-                                 (o.legacy_local ? "_legacy_local" : "") + ".bin";
+                                 (o.policy == WR_POLICY_LEGACY_LOCAL    ? "_legacy_local"
+                                  : o.policy == WR_POLICY_GRIDWIDE_CTA32 ? "_cta32"
+                                                                         : "") +
+                                 ".bin";
         if (wr_write_final_states(cells.data(), num_cells, o.grid_dim, file.c_str()) != WR_SUCCESS)
             return finish(false, comparison_done);
         printf("final states: %s (%d packed records, %zu bytes each)\n", file.c_str(), num_cells,

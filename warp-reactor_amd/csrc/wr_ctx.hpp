@@ -64,9 +64,12 @@ struct Ctl {
     int jac_age_max;      // Jacobian reuse: accepted steps per J (1: the reference)
     unsigned long long mixed_lu_fallbacks;      // attempts whose FP32 factors were singular
     unsigned long long mixed_refine_fallbacks;  // stage solves whose refinement did not converge
-    // legacy-local policy: (failing cell << 32 | code) of the step's lowest
-    // failing cell, all ones when none failed
+    // legacy-local and gridwide-cta32 policies: (failing cell << 32 | code) of
+    // the step's lowest failing cell (or tile's first cell), all ones when none
+    // failed
     unsigned long long local_fail_key;
+    // gridwide-cta32 policy: density-driver updates as (dt * driver) / tff
+    int mojo_order;
     int bin_count[16];  // cells per attempt-count bin, for the queue order
     int bin_used[16];   // queue positions already handed to blocks, per bin
 };
@@ -157,8 +160,10 @@ struct wr_context {
     hipEvent_t adv_start[kChunk] = {}, adv_stop[kChunk] = {};
     double adv_ms = 0.0;
     int adv_launches = 0;
-    // driver policy of wr_run: WR_POLICY_GRIDWIDE (default) or WR_POLICY_LEGACY_LOCAL
+    // driver policy of wr_run: WR_POLICY_GRIDWIDE (default), WR_POLICY_LEGACY_LOCAL
+    // or WR_POLICY_GRIDWIDE_CTA32
     int policy = 0;
+    int cta_blocks = 0;  // persistent grid of the gridwide-cta32 kernel (0: not sized yet)
     size_t cell_bytes = 0;           // bytes of cells_mem
     void* local_backup = nullptr;    // cells at the start of a legacy-local step
     int q_blocks_local = 0;          // persistent grid of the legacy-local kernel

@@ -13,12 +13,16 @@
 //              the cell's full adaptive ROS2S advance (7736-7785).
 //   k_commit   one thread: failure / zero-integrated termination, grid time
 //              and global step bookkeeping (7943-7958).
+// The production advance is the queue-fed k_advance_q (wr_queue.hpp, with
+// wave-cooperative cells from wr_coop.hpp); the Mojo front end's
+// gridwide-cta32 policy replaces the advance by k_advance_cta (wr_cta.hpp).
 // Every kernel returns immediately once the device control block says the
 // run is done, so the host enqueues several steps ahead and polls one flag.
 #include "wr_ctx.hpp"
 #include "wr_device.hpp"
 #include "wr_queue.hpp"
 #include "wr_coop.hpp"
+#include "wr_cta.hpp"
 
 #include <float.h>
 #include <stdio.h>
@@ -149,7 +153,11 @@ __global__ __launch_bounds__(kPrepBlock) void k_prepare(DevCells s, Ctl* ctl, do
                     set_failure(ctl, kBadInputs, c);
                 } else if (dt < 10.0) {
                     const double tff = dt / 1.0e-1;
-                    driver += dt * (driver / tff);
+                    // gridwide-cta32 evaluates (dt * driver) / tff (reproducer.mojo:2858)
+                    if (ctl->mojo_order)
+                        driver += dt * driver / tff;
+                    else
+                        driver += dt * (driver / tff);
                 } else {
                     cand = dt;
                     cand_cell = c;
@@ -210,6 +218,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan(Ctl* ctl, const double* par
     ctl->queue = 0;
     ctl->queue_solo = 0;
     ctl->queue_coop = 0;
+    ctl->local_fail_key = ~0ull;
     const int k = ctl->step - ctl->step0;
     if (ctl->dt_hist && k >= 0 && k < ctl->hist_cap) ctl->dt_hist[k] = dt;
 }
@@ -357,6 +366,12 @@ __global__ void k_commit(Ctl* ctl) {
     for (int b = 0; b < kCostBins; ++b) {
         ctl->bin_count[b] = 0;
         ctl->bin_used[b] = 0;
+    }
+    // gridwide-cta32: the lowest failing tile (its first cell) is reported
+    const unsigned long long key = ctl->local_fail_key;
+    if (key != ~0ull) {
+        ctl->failure = static_cast<int>(key & 0xffffffffull);
+        ctl->failed_cell = static_cast<int>(key >> 32);
     }
     if (ctl->failure != kSuccess) {
         ctl->done = 1;
@@ -691,8 +706,9 @@ int wr_upload_cells(wr_context* ctx, const wr_collapse_state* cells, int num_cel
     ctx->n = num_cells;
     ctx->cells.n = num_cells;
     if (num_cells == 0) return WR_SUCCESS;
+    // host or device memory: the copy kind follows from unified addressing
     WR_HIP(hipMemcpyAsync(ctx->staging, cells, num_cells * sizeof(wr_collapse_state),
-                          hipMemcpyHostToDevice, ctx->stream),
+                          hipMemcpyDefault, ctx->stream),
            "hipMemcpy(cells to device)");
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(num_cells, 256)), dim3(256), 0, ctx->stream,
                        ctx->staging, ctx->cells);
@@ -710,7 +726,7 @@ int wr_download_cells(wr_context* ctx, wr_collapse_state* cells, int num_cells) 
                        ctx->staging, ctx->cells);
     WR_HIP(hipGetLastError(), "k_gather launch");
     WR_HIP(hipMemcpyAsync(cells, ctx->staging, num_cells * sizeof(wr_collapse_state),
-                          hipMemcpyDeviceToHost, ctx->stream),
+                          hipMemcpyDefault, ctx->stream),
            "hipMemcpy(cells to host)");
     WR_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return WR_SUCCESS;
@@ -765,6 +781,8 @@ static int init_ctl(wr_context* ctx, int perturb, int max_steps, int completed, 
     h.lu_sweeps = ctx->lu_sweeps;
     h.solo_lanes = ctx->q_solo;
     h.jac_age_max = ctx->jac_age_max;
+    h.local_fail_key = ~0ull;
+    h.mojo_order = ctx->policy == WR_POLICY_GRIDWIDE_CTA32 ? 1 : 0;
     if (coop_active(ctx)) coop_resolved(ctx, &h.coop_cells, &h.coop_waves);
     *ctx->ctl_host = h;
     WR_HIP(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, ctx->stream),
@@ -814,6 +832,21 @@ static int queue_grid(wr_context* ctx) {
     return WR_SUCCESS;
 }
 
+// Persistent grid of the gridwide-cta32 kernel: every resident wave slot (at
+// most one wave per 64 cells).
+static int cta_grid(wr_context* ctx) {
+    if (ctx->cta_blocks > 0) return WR_SUCCESS;
+    int cus = 0, per_cu = 0;
+    WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+           "hipDeviceGetAttribute(CU count)");
+    WR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_advance_cta, kCtaBlock, 0),
+           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    const int blocks = cus * (per_cu < 1 ? 1 : per_cu);
+    const int need = blocks_for(ctx->cap, kCtaBlock);
+    ctx->cta_blocks = blocks > need ? need : (blocks < 1 ? 1 : blocks);
+    return WR_SUCCESS;
+}
+
 static int launch_advance(wr_context* ctx) {
     if (ctx->lu_mode != 0 && ctx->advance_kernel != 1)
         return fail(WR_ERR_ARGUMENT,
@@ -822,7 +855,16 @@ static int launch_advance(wr_context* ctx) {
         return fail(WR_ERR_ARGUMENT,
                     "Jacobian reuse (wr_set_jacobian_reuse) needs the queue-fed advance kernel "
                     "with FP64 factors");
-    if (ctx->advance_kernel == 1) {
+    if (ctx->policy == WR_POLICY_GRIDWIDE_CTA32) {
+        if (ctx->lu_mode != 0 || ctx->jac_age_max > 1)
+            return fail(WR_ERR_ARGUMENT,
+                        "the gridwide-cta32 policy runs the reference integrator "
+                        "(wr_set_lu_precision 0, wr_set_jacobian_reuse 1)");
+        const int st = cta_grid(ctx);
+        if (st != WR_SUCCESS) return st;
+        hipLaunchKernelGGL(k_advance_cta, dim3(ctx->cta_blocks), dim3(kCtaBlock), 0, ctx->stream,
+                           ctx->cells, ctx->ctl, 0);
+    } else if (ctx->advance_kernel == 1) {
         const int st = queue_grid(ctx);
         if (st != WR_SUCCESS) return st;
         const int nb = blocks_for(ctx->n, kOrderBlock);
@@ -857,6 +899,8 @@ static void fill_info(wr_context* ctx) {
     i.failed_step = h.failed_step;
     i.failed_phase = h.failed_phase;
     i.failed_cell = h.failed_cell;
+    i.limiting_cell = h.limiting_cell;
+    i.reserved_ = 0;
     i.grid_time = h.grid_time;
     i.cell_advances = h.cell_advances;
     // steps whose dt_grid was planned: completed steps, plus one if the run
@@ -889,7 +933,8 @@ static int enqueue_steps(wr_context* ctx, int todo, bool timed) {
 static int chunk_graph(wr_context* ctx) {
     const long long kernel_key =
         ctx->advance_kernel + 2 * ctx->lu_mode + 4 * (ctx->jac_age_max > 1) +
-        8LL * (ctx->q_threads + 1024LL * (ctx->q_per_cu + 64LL * (ctx->q_solo + 128LL * coop_active(ctx))));
+        8LL * (ctx->q_threads + 1024LL * (ctx->q_per_cu + 64LL * (ctx->q_solo + 128LL * coop_active(ctx)))) +
+        (1LL << 40) * ctx->policy;
     if (ctx->chunk_exec && ctx->graph_n == ctx->n && ctx->graph_kernel == kernel_key)
         return WR_SUCCESS;
     if (ctx->chunk_exec) {
@@ -897,6 +942,7 @@ static int chunk_graph(wr_context* ctx) {
         ctx->chunk_exec = nullptr;
     }
     int st = queue_grid(ctx);  // occupancy queries stay outside the capture
+    if (st == WR_SUCCESS && ctx->policy == WR_POLICY_GRIDWIDE_CTA32) st = cta_grid(ctx);
     if (st != WR_SUCCESS) return st;
     hipGraph_t g = nullptr;
     WR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
@@ -1085,8 +1131,11 @@ int wr_dt_history(const wr_context* ctx, double* out, int max_entries, int* coun
 }
 
 int wr_set_policy(wr_context* ctx, int policy) {
-    if (!ctx || (policy != WR_POLICY_GRIDWIDE && policy != WR_POLICY_LEGACY_LOCAL))
-        return fail(WR_ERR_ARGUMENT, "wr_set_policy: policy must be 0 (gridwide) or 1 (legacy-local)");
+    if (!ctx || (policy != WR_POLICY_GRIDWIDE && policy != WR_POLICY_LEGACY_LOCAL &&
+                 policy != WR_POLICY_GRIDWIDE_CTA32))
+        return fail(WR_ERR_ARGUMENT,
+                    "wr_set_policy: policy must be 0 (gridwide), 1 (legacy-local) or 2 "
+                    "(gridwide-cta32)");
     ctx->policy = policy;
     return WR_SUCCESS;
 }
@@ -1241,6 +1290,9 @@ int wr_advance(wr_context* ctx, int completed_global_steps, double next_grid_tim
     if (ctx->n > 0 && (st = launch_advance(ctx)) != WR_SUCCESS) return st;
     if ((st = read_ctl(ctx)) != WR_SUCCESS) return st;
     if (integrated) *integrated = ctx->ctl_host->integrated;
+    // gridwide-cta32: the lowest failing tile's code
+    const unsigned long long key = ctx->ctl_host->local_fail_key;
+    if (key != ~0ull) return static_cast<int>(key & 0xffffffffull);
     return ctx->ctl_host->failure;
 }
 

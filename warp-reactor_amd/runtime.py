@@ -44,7 +44,8 @@ class RunInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("stop_reason", ctypes.c_int32),
                 ("completed_global_steps", ctypes.c_int32), ("failed_step", ctypes.c_int32),
                 ("failed_phase", ctypes.c_int32), ("failed_cell", ctypes.c_int32),
-                ("grid_time", ctypes.c_double), ("cell_advances", ctypes.c_uint64)]
+                ("grid_time", ctypes.c_double), ("cell_advances", ctypes.c_uint64),
+                ("limiting_cell", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 class ComparisonSummary(ctypes.Structure):
@@ -268,6 +269,18 @@ class Context:
         assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
         _check(self._lib.wr_download_cells(self._h, _ptr(cells), len(cells)), "wr_download_cells")
 
+    def upload_ptr(self, ptr, n):
+        """Upload n cells from a raw address (host or device memory of the
+        context's device, e.g. a torch tensor's data_ptr(); its data must be
+        complete before the call)."""
+        _check(self._lib.wr_upload_cells(self._h, ctypes.c_void_p(int(ptr)), int(n)),
+               "wr_upload_cells")
+        self._n = int(n)
+
+    def download_ptr(self, ptr, n):
+        _check(self._lib.wr_download_cells(self._h, ctypes.c_void_p(int(ptr)), int(n)),
+               "wr_download_cells")
+
     def run(self, perturb=True, max_steps=1000, completed=0, grid_time=0.0, first_step=None):
         """Outer loop on resident cells; returns the run info as a dict.
         first_step: loop index of the first step (None: the completed count,
@@ -291,10 +304,11 @@ class Context:
         return out[:n.value].copy()
 
     def set_policy(self, policy):
-        """Driver policy of subsequent runs: "gridwide" (default, run_cells_cuda)
-        or "legacy-local" (the reference's CPU driver: each cell steps with its
-        own free-fall dt)."""
-        code = {"gridwide": 0, "legacy-local": 1}.get(policy, policy)
+        """Driver policy of subsequent runs: "gridwide" (default, run_cells_cuda),
+        "legacy-local" (the reference's CPU driver: each cell steps with its
+        own free-fall dt) or "gridwide-cta32" (the Mojo front end's policy:
+        one step controller per 32-cell tile)."""
+        code = {"gridwide": 0, "legacy-local": 1, "gridwide-cta32": 2}.get(policy, policy)
         _check(self._lib.wr_set_policy(self._h, int(code)), "wr_set_policy")
 
     def set_jacobian_reuse(self, age_max):
@@ -449,6 +463,14 @@ class Context:
         g = ctypes.c_ulonglong()
         _check(self._lib.wr_adjoint_generation(self._h, ctypes.byref(g)), "wr_adjoint_generation")
         return g.value
+
+    def adjoint_backward_ptr(self, gN, g0, mbar, generation):
+        """wr_adjoint_backward_gen on raw addresses (host or device memory of
+        the context's device): gN, g0 [18][n], mbar [R][n] float64."""
+        r = self._lib.wr_adjoint_backward_gen(self._h, int(generation), ctypes.c_void_p(int(gN)),
+                                              ctypes.c_void_p(int(g0)), ctypes.c_void_p(int(mbar)))
+        if r != SUCCESS:
+            _check(r, "wr_adjoint_backward")
 
     def adjoint_backward(self, gN, generation=None):
         """dL/d(first state), dL/dM per cell; `generation` (adjoint_generation()
