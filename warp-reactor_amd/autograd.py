@@ -19,7 +19,10 @@ from T), so its gradient is zero.  PyTorch is only the binding: the compute
 runs in the HIP library.  Tensors on the GPU stay there: the cells, seeds and
 gradients cross the boundary as device addresses (the library copies them on
 its own stream after torch's work on them is complete); CPU tensors cross as
-float64 host arrays.
+float64 host arrays.  Device addresses are only meaningful within one HIP
+runtime: torch's wheel carries its own libamdhip64, so torch must be imported
+(it maps its runtime) before the library is loaded, which then binds to that
+same runtime; the device path checks this and raises otherwise.
 """
 from __future__ import annotations
 
@@ -75,6 +78,22 @@ def state_of(cells):
                            c["current"]["rho"][None], c["density_driver"][None]])
 
 
+def _hip_runtimes():
+    """Distinct HIP runtime libraries mapped into this process (Linux)."""
+    try:
+        with open("/proc/self/maps") as f:
+            return {l.split()[-1] for l in f if "libamdhip64" in l}
+    except OSError:
+        return set()
+
+
+def _require_one_hip_runtime():
+    if len(_hip_runtimes()) > 1:
+        raise RuntimeError(
+            "torch and the HIP library run on different HIP runtimes (the library was loaded "
+            "before torch): import torch before loading warp-reactor_amd to pass GPU tensors")
+
+
 class CollapseAdjoint(torch.autograd.Function):
     """sN = collapse(theta, s0) on a wr.Context; gradients by the discrete adjoint."""
 
@@ -87,6 +106,7 @@ class CollapseAdjoint(torch.autograd.Function):
         fctx.M = M
         fctx.theta_device = theta.device
         if s0.is_cuda:
+            _require_one_hip_runtime()
             cells = cells_tensor_from_state(s0.detach().to(torch.float64)).contiguous()
             torch.cuda.synchronize(s0.device)  # the records are complete before the copy
             context.upload_ptr(cells.data_ptr(), cells.shape[0])
