@@ -18,7 +18,16 @@
  * advance semantics, and the grid-1 known answer (438 global steps).  Closed
  * forms pin the rest: SplitMix64 against its published test vector, the
  * free-fall time formula, mass/charge conservation of the RHS, LU against
- * brute-force permutation checks.  No function is "parity unpinned".
+ * brute-force permutation checks.
+ *
+ * or_run_cta restates the Mojo front end's gridwide-cta32 policy
+ * (reproducer.mojo), for which no toolchain is here: it is pinned by what
+ * the policy's definition fixes (tests/test_cta.py: one-cell tiles equal the
+ * pinned gridwide run, identical cells share one path, tile invariants, the
+ * reference comparator), except the operation order of its two
+ * density-driver updates, (dt * driver) / tff, which follows the Mojo source
+ * by reading: "parity unpinned" for that order.  Every other function is
+ * pinned as above.
  *
  * Build: oracle/Makefile, gcc -O2 -ffp-contract=off (no FMA contraction; x86-64
  * SSE2 arithmetic is IEEE double).

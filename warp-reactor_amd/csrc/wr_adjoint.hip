@@ -145,16 +145,29 @@ __global__ void k_adj_dts(const Ctl* ctl, double* dts, int step) {
 // ---- one accepted ROS2S step at a fixed h -------------------------------------------------
 // The attempt of the integrator (reproducer.cpp:7260-7302), same operations in the
 // same order; J, LU, k1..k3, u2, u3 stay in the workspace for the adjoint.
+// unit: every multiplier is exactly 1, and the production network gives
+// f_theta and J_theta bit for bit (J_theta = J - (+0)); its stage vectors use
+// the W slots of the workspace, which a forward step does not touch.
 __device__ __forceinline__ int adj_substep(double (&y)[kNeqs], double h, const double* M,
-                                           double* w, int st, uint64_t& piv) {
+                                           double* w, int st, uint64_t& piv, bool unit) {
     using C = Ros2sCoef;
-    ad_fj_st(y, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVF) * st, w + kWJ * st);
     ThreadWs tw;
     tw.J = w + kWJ * st;
     tw.LU = w + kWLU * st;
-    tw.vec = nullptr;
-    tw.stage = nullptr;
+    tw.vec = w + kWW * st;
+    tw.stage = w + kWSTG * st;
     tw.stride = st;
+    static_assert(kVecLen <= kWV - kWW, "production stage vectors fit in the W slots");
+    static_assert(kStageSlots <= AD_FWD_NSLOT, "production staging fits in the slots");
+    if (unit) {
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) WR_AT(tw.vec, kVecY + i, st) = y[i];
+        ws_jac_rhs(tw);
+#pragma unroll
+        for (int i = 0; i < kNeqs; ++i) w[(kWV + kVF + i) * st] = WR_AT(tw.vec, kVecF0 + i, st);
+    } else {
+        ad_fj_st(y, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVF) * st, w + kWJ * st);
+    }
     const double fac = 1.0 / (h * C::gamma);
     if (lu_left(tw, fac, piv) != 0) return kLuError;
     double v[kNeqs], u[kNeqs], k1[kNeqs], k2[kNeqs];
@@ -181,9 +194,17 @@ __device__ __forceinline__ int adj_substep(double (&y)[kNeqs], double h, const d
             }
             w[(kWV + uf + i) * st] = u[i];
         }
-        ad_f_st(u, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVDY) * st, nullptr);
+        if (unit) {
 #pragma unroll
-        for (int i = 0; i < kNeqs; ++i) v[i] += w[(kWV + kVDY + i) * st];
+            for (int i = 0; i < kNeqs; ++i) WR_AT(tw.vec, kVecYnew + i, st) = u[i];
+            ws_rhs(tw);
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) v[i] += WR_AT(tw.vec, kVecDy + i, st);
+        } else {
+            ad_f_st(u, 1, M, 1, w + kWSTG * st, st, w + (kWV + kVDY) * st, nullptr);
+#pragma unroll
+            for (int i = 0; i < kNeqs; ++i) v[i] += w[(kWV + kVDY + i) * st];
+        }
         lu_solve_ws(tw, piv, v);
     }
 #pragma unroll
@@ -400,7 +421,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_replay(AdjDev a, const double
         const int k1 = a.hoff[static_cast<size_t>(step + 1) * a.cap + c];
         for (int k = k0; k < k1; ++k) {
             uint64_t piv;
-            if (adj_substep(y, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, w, st, piv) !=
+            // the replay's multipliers are not the recording's: always f_theta
+            if (adj_substep(y, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, w, st, piv, false) !=
                 kSuccess)
                 adj_error(a.err, kLuError);
         }
@@ -448,7 +470,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __res
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) slot[(kSY + i) * a.cap] = y[i];
         uint64_t piv;
-        if (adj_substep(y, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, w, nA, piv) != kSuccess)
+        if (adj_substep(y, a.htape[static_cast<size_t>(k) * a.cap + c], a.M, w, nA, piv, a.unit) !=
+            kSuccess)
             adj_error(a.err, kLuError);
         for (int e = 0; e < kNeqs * kNeqs; ++e) slot[(kSLU + e) * a.cap] = w[(kWLU + e) * nA];
 #pragma unroll
