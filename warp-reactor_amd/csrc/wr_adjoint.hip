@@ -574,6 +574,10 @@ struct AdjointState {
     std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
+    double* sub2 = nullptr;   // the other buffer: the next window's substeps are
+    size_t sub2_elems = 0;    // recomputed while the current window is reversed
+    hipStream_t steps_stream = nullptr;
+    hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, go = nullptr;
     bool recorded = false;
     // number of successful recordings on this context: a backward pass names
     // the recording it differentiates (wr_adjoint_backward_gen)
@@ -609,6 +613,13 @@ void wr::detail::adjoint_destroy(wr_context* ctx) {
     (void)hipFree(a->Mr);
     (void)hipFree(a->ws);
     (void)hipFree(a->sub);
+    (void)hipFree(a->sub2);
+    for (int b = 0; b < 2; ++b) {
+        if (a->ready[b]) (void)hipEventDestroy(a->ready[b]);
+        if (a->freed[b]) (void)hipEventDestroy(a->freed[b]);
+    }
+    if (a->go) (void)hipEventDestroy(a->go);
+    if (a->steps_stream) (void)hipStreamDestroy(a->steps_stream);
     (void)hipFree(a->err);
     (void)hipFree(a->io);
     if (a->ev0) (void)hipEventDestroy(a->ev0);
@@ -872,9 +883,13 @@ int wr_adjoint_replay(wr_context* ctx, const double* s0, const double* multiplie
     return code == 0 ? kSuccess : kLuError;
 }
 
-// Memory for the stored substeps and the step-recompute workspace of one
-// window of steps; windows are taken from the last step backwards.
-constexpr size_t kWindowBudgetBytes = size_t(24) << 30;
+// Memory for the stored substeps of one window of steps (two windows are
+// held at once); windows are taken from the last step backwards, at least
+// kMinWindows of them when there are that many steps, so that recomputing a
+// window's steps (k_adj_steps, second stream) overlaps the reverse sweep of
+// the window after it (k_adj_reverse, the context's stream).
+constexpr size_t kWindowBudgetBytes = size_t(12) << 30;
+constexpr int kMinWindows = 4;
 
 int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* mbar) {
     if (!ctx || !ctx->adj || !ctx->adj->recorded || !gN || !g0 || !mbar)
@@ -903,41 +918,71 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
         lpw = static_cast<int>((n + simds - 1) / simds);
         lpw = lpw < 32 ? 32 : (lpw > kAdjBlock ? kAdjBlock : lpw);
     }
-    if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
-    int s1 = a->nsteps;
-    while (s1 > 0) {
-        // the longest window ending at s1 that fits the budget (at least one step)
-        int s0 = s1 - 1;
-        auto bytes = [&](int lo) {
-            int most = 0;
+    // windows from the end: the longest ending at s1 that fits the budget (at
+    // least one step), at most ceil(nsteps / kMinWindows) steps long
+    struct Window {
+        int s0, s1, most;
+    };
+    std::vector<Window> wins;
+    size_t most_all = 1;
+    const int span = (a->nsteps + kMinWindows - 1) / kMinWindows;
+    for (int s1 = a->nsteps; s1 > 0;) {
+        auto most_of = [&](int lo) {
+            int most = 1;
             for (int c = 0; c < n; ++c) {
                 const int k = a->hoff_h[static_cast<size_t>(s1) * cap + c] -
                               a->hoff_h[static_cast<size_t>(lo) * cap + c];
                 if (k > most) most = k;
             }
-            return static_cast<size_t>(most) * kSub * cap * sizeof(double);
+            return most;
         };
-        while (s0 > 0 && bytes(s0 - 1) <= kWindowBudgetBytes &&
-               (a->window_max <= 0 || s1 - (s0 - 1) <= a->window_max))
+        int s0 = s1 - 1;
+        while (s0 > 0 &&
+               static_cast<size_t>(most_of(s0 - 1)) * kSub * cap * sizeof(double) <=
+                   kWindowBudgetBytes &&
+               s1 - (s0 - 1) <= span && (a->window_max <= 0 || s1 - (s0 - 1) <= a->window_max))
             --s0;
-        int most = 1;
-        for (int c = 0; c < n; ++c) {
-            const int k = a->hoff_h[static_cast<size_t>(s1) * cap + c] -
-                          a->hoff_h[static_cast<size_t>(s0) * cap + c];
-            if (k > most) most = k;
-        }
-        const int nA = n * (s1 - s0);
-        if ((st = ensure_buf(&a->sub, &a->sub_elems, static_cast<size_t>(most) * kSub * cap,
-                             "hipMalloc(adjoint substeps)")) != WR_SUCCESS)
-            return st;
-        hipLaunchKernelGGL(k_adj_steps, dim3(blocks_for(nA, kAdjBlock)), dim3(kAdjBlock), 0,
-                           ctx->stream, d, a->sub, n, s0, s1, 0);
-        WR_HIP(hipGetLastError(), "k_adj_steps launch");
-        hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, lpw)), dim3(kAdjBlock), 0,
-                           ctx->stream, d, a->sub, d_lam, n, s0, s1, s1 == a->nsteps ? 1 : 0, 0,
-                           lpw);
-        WR_HIP(hipGetLastError(), "k_adj_reverse launch");
+        const int most = most_of(s0);
+        wins.push_back({s0, s1, most});
+        if (static_cast<size_t>(most) > most_all) most_all = most;
         s1 = s0;
+    }
+    const size_t need = most_all * kSub * cap;
+    if ((st = ensure_buf(&a->sub, &a->sub_elems, need, "hipMalloc(adjoint substeps)")) != WR_SUCCESS)
+        return st;
+    if (wins.size() > 1 &&
+        (st = ensure_buf(&a->sub2, &a->sub2_elems, need, "hipMalloc(adjoint substeps)")) !=
+            WR_SUCCESS)
+        return st;
+    if (!a->steps_stream) {
+        WR_HIP(hipStreamCreate(&a->steps_stream), "hipStreamCreate");
+        for (int b = 0; b < 2; ++b) {
+            WR_HIP(hipEventCreateWithFlags(&a->ready[b], hipEventDisableTiming), "hipEventCreate");
+            WR_HIP(hipEventCreateWithFlags(&a->freed[b], hipEventDisableTiming), "hipEventCreate");
+        }
+        WR_HIP(hipEventCreateWithFlags(&a->go, hipEventDisableTiming), "hipEventCreate");
+    }
+    double* const buf[2] = {a->sub, a->sub2};
+    if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
+    // the recompute starts after the status word is cleared
+    WR_HIP(hipEventRecord(a->go, ctx->stream), "hipEventRecord");
+    WR_HIP(hipStreamWaitEvent(a->steps_stream, a->go, 0), "hipStreamWaitEvent");
+    for (size_t i = 0; i < wins.size(); ++i) {
+        const Window& wdw = wins[i];
+        const int b = static_cast<int>(i & 1);
+        // buffer b is free once the reverse sweep of window i - 2 has read it
+        if (i >= 2) WR_HIP(hipStreamWaitEvent(a->steps_stream, a->freed[b], 0), "hipStreamWaitEvent");
+        const int nA = n * (wdw.s1 - wdw.s0);
+        hipLaunchKernelGGL(k_adj_steps, dim3(blocks_for(nA, kAdjBlock)), dim3(kAdjBlock), 0,
+                           a->steps_stream, d, buf[b], n, wdw.s0, wdw.s1, 0);
+        WR_HIP(hipGetLastError(), "k_adj_steps launch");
+        WR_HIP(hipEventRecord(a->ready[b], a->steps_stream), "hipEventRecord");
+        WR_HIP(hipStreamWaitEvent(ctx->stream, a->ready[b], 0), "hipStreamWaitEvent");
+        hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, lpw)), dim3(kAdjBlock), 0,
+                           ctx->stream, d, buf[b], d_lam, n, wdw.s0, wdw.s1,
+                           wdw.s1 == a->nsteps ? 1 : 0, 0, lpw);
+        WR_HIP(hipGetLastError(), "k_adj_reverse launch");
+        WR_HIP(hipEventRecord(a->freed[b], ctx->stream), "hipEventRecord");
     }
     hipLaunchKernelGGL(k_adj_mbar, dim3(blocks_for(n, kAdjBlock)), dim3(kAdjBlock), 0, ctx->stream,
                        d, d_mb, n);
