@@ -10,13 +10,15 @@ First three device-to-device copies of 1 GiB (hipMemcpy: the runtime's copy
 kernel, 3 GiB read and 3 GiB written over its dispatches) calibrate what the
 counters read for a known number of bytes; then one collapse run of the workload on
 the library (make_collapse_state cells, perturbations, the given outer steps,
-the queue-fed advance kernel with the given launch shape, HIP graphs).
-Needs no torch."""
+the queue-fed advance kernel with the given launch shape, HIP graphs).  With
+--timed (not under the profiler) one outer step loads the kernels first and
+the run's wall time is printed.  Needs no torch."""
 import argparse
 import ctypes
 import importlib
 import os
 import sys
+import time
 
 REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, REPO)
@@ -50,8 +52,10 @@ def main():
     ap.add_argument("--per-cu", type=int, default=0)
     ap.add_argument("--solo", type=int, default=0)
     ap.add_argument("--coop", type=int, default=-1)
+    ap.add_argument("--timed", action="store_true")
     args = ap.parse_args()
-    calibration_copies()
+    if not args.timed:
+        calibration_copies()
     wr = importlib.import_module("warp-reactor_amd")
     n = args.grid ** 3
     init = wr.make_collapse_states(n)
@@ -59,10 +63,16 @@ def main():
         ctx.set_queue_launch(args.threads, args.per_cu, args.solo)
         if args.coop >= 0:
             ctx.set_coop(args.coop, args.coop)
+        if args.timed:
+            ctx.upload(init)
+            ctx.run(perturb=True, max_steps=1)  # loads the kernels
         ctx.upload(init)
+        t0 = time.perf_counter()
         info = ctx.run(perturb=True, max_steps=args.max_outer)
+        el = time.perf_counter() - t0
+        lanes = ctx.queue_launch()[3] * args.threads
     print(f"run: {info['completed_global_steps']} outer steps, {info['cell_advances']} "
-          f"cell-advances, status {info['status']}", flush=True)
+          f"cell-advances, status {info['status']}, {el:.3f} s, {lanes} resident lanes", flush=True)
 
 
 if __name__ == "__main__":
