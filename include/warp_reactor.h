@@ -349,8 +349,7 @@ int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double 
  * Results do not depend on it. */
 int wr_adjoint_set_window(wr_context *ctx, int max_steps);
 /* (new) Cells per 64-lane wave of the backward pass's reverse sweep, 1..64;
- * 0 (the default) picks one wave per SIMD when the grid has fewer than 64
- * cells per SIMD (at least 32 cells per wave).  Results do not depend on it. */
+ * 0 (the default) means 64.  Results do not depend on it. */
 int wr_adjoint_set_lanes(wr_context *ctx, int cells_per_wave);
 /* Device time (HIP events on the context's stream) of the last record,
  * replay or backward call, ms. */

@@ -489,9 +489,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __res
 // Phase B, one thread per cell: the window's steps in reverse from the stored
 // substeps.  lam [18][cap] carries dL/d(state) across windows; dL/dM
 // accumulates in the workspace (cleared when `first`).  Each 64-lane wave
-// (one block) takes `lpw` cells on its first lanes: a wave's sweep is bound by
-// the latency of its own serial chain, not by its lane count, so on grids with
-// fewer than 64 cells per SIMD, fewer cells per wave put a wave on every SIMD.
+// (one block) takes `lpw` cells on its first lanes (64 by default; fewer put
+// waves on more SIMDs of a small grid).
 __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const double* __restrict__ sub,
                                                            double* __restrict__ lamv, int n, int s0,
                                                            int s1, int first, int zero, int lpw) {
@@ -570,7 +569,7 @@ __global__ void k_adj_mbar(AdjDev a, double* __restrict__ mbar, int n) {
 struct AdjointState {
     int nsteps = 0, hcap = 0, cap = 0, n = 0;
     int window_max = 0;       // most steps per backward window (0: by memory budget)
-    int lanes_per_wave = 0;   // cells per wave of the reverse sweep (0: by grid size)
+    int lanes_per_wave = 0;   // cells per wave of the reverse sweep (0: 64)
     std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
@@ -904,20 +903,10 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     if ((st = put_fields(ctx, d_lam, gN, kNF)) != WR_SUCCESS) return st;
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
     const AdjDev d = dev_view(a, a->M);
-    // cells per wave of the reverse sweep: one wave per SIMD (four per CU)
-    // when the grid has fewer than 64 cells per SIMD, at least 32 per wave
-    // (at 32^3, 16 cells per wave -- two waves per SIMD -- took 1.65x longer)
-    int cus = 0;
-    WR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
-           "hipDeviceGetAttribute(CU count)");
-    int lpw = kAdjBlock;
-    if (a->lanes_per_wave > 0) {
-        lpw = a->lanes_per_wave;
-    } else if (cus > 0) {
-        const long long simds = 4LL * cus;
-        lpw = static_cast<int>((n + simds - 1) / simds);
-        lpw = lpw < 32 ? 32 : (lpw > kAdjBlock ? kAdjBlock : lpw);
-    }
+    // cells per wave of the reverse sweep: full waves by default (at 32^3,
+    // 512 waves: 183 ms against 192 ms with 32 cells per wave on every SIMD;
+    // the step recompute of the next window uses the SIMDs left free)
+    const int lpw = a->lanes_per_wave > 0 ? a->lanes_per_wave : kAdjBlock;
     // windows from the end: the longest ending at s1 that fits the budget (at
     // least one step), at most ceil(nsteps / kMinWindows) steps long
     struct Window {
