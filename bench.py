@@ -389,6 +389,8 @@ def main():
         line["lognormal128"] = lognormal128_measure(wr, ctx_shape)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_measure(wr)
+        if args.grid == 64 and args.recipe == "reference" and args.max_outer >= 100:
+            line["adjoint_default_run"] = adjoint_default_run_measure(wr)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(init, dts, args.cpu_budget)
     kname = line["roofline"]["kernel"]
@@ -593,6 +595,59 @@ def cta32_measure(ctx, init, max_outer, ref_cells, n, wr):
                 "max_thermodynamic_rel_error", "max_deuterium_species_rel_error")}}
 
 
+def adjoint_default_run_measure(wr, grid=64, nsteps=100, hcap=4096, directions=2, delta=1e-6):
+    """The discrete adjoint of the first `nsteps` outer steps of the default
+    run (reference initial state, density perturbations at steps 20, 40, ...):
+    record and backward times, and the gradient of sum(log T) against central
+    differences of the frozen-map replay along seeded directions."""
+    n = grid ** 3
+    init = wr.make_collapse_states(n)
+    R = wr.adjoint_ntheta()
+    with wr.Context(n) as ctx:
+        ctx.upload(init)
+        ctx.run(perturb=True, max_steps=1)  # loads the kernels
+        ctx.upload(init)
+        t0 = time.perf_counter()
+        ctx.run(perturb=True, max_steps=nsteps)  # the production forward of the same steps
+        fwd = time.perf_counter() - t0
+        ctx.upload(init)
+        t0 = time.perf_counter()
+        ctx.adjoint_record(nsteps, np.ones(R), hcap=hcap, perturb=True)
+        rec = time.perf_counter() - t0
+        final = np.zeros(n, dtype=wr.CELL_DTYPE)
+        ctx.download(final)
+        attempts = int(final["stats"]["internal_steps"].sum())
+        gN = np.zeros((18, n))
+        gN[14] = 1.0 / final["current"]["T"]  # d sum(log T) / dT
+        ctx.adjoint_backward(gN)  # allocates the window buffers
+        t0 = time.perf_counter()
+        g0, mbar = ctx.adjoint_backward(gN)
+        bwd = time.perf_counter() - t0
+        s0 = np.concatenate([init["current"]["xn"].T, init["current"]["T"][None],
+                             init["current"]["e"][None], init["current"]["rho"][None],
+                             init["density_driver"][None]])
+        rng = np.random.default_rng(5)
+        errs = []
+        for _ in range(directions):
+            v = rng.standard_normal(s0.shape) * np.abs(s0)
+            v[15] = 0.0
+            dth = rng.standard_normal(R)
+            lp = np.sum(np.log(ctx.adjoint_replay(s0 + delta * v, np.exp(delta * dth))[14]))
+            lm = np.sum(np.log(ctx.adjoint_replay(s0 - delta * v, np.exp(-delta * dth))[14]))
+            fd = (lp - lm) / (2 * delta)
+            an = float(np.sum(g0 * v) + mbar.sum(axis=1) @ dth)
+            errs.append(abs(fd - an) / abs(an))
+    print(f"# adjoint of the default run: {nsteps} steps, forward {fwd:.3f} s, record {rec:.3f} s, "
+          f"backward {bwd:.3f} s, FD max rel err {max(errs):.2e}", file=sys.stderr, flush=True)
+    return {"workload": f"{grid}^3 = {n} cells, reference initial state, the first {nsteps} outer "
+                        "steps of the default run with its density perturbations, loss "
+                        "sum(log T), 45 log-rate multipliers + 18 state fields per cell",
+            "forward_seconds": fwd, "record_seconds": rec, "backward_seconds": bwd,
+            "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd,
+            "attempts": attempts,
+            "fd_check": {"directions": directions, "delta": delta, "max_rel_err": max(errs)}}
+
+
 def adjoint_measure(wr, grid=32, nsteps=20, seed=2, directions=8, delta=1e-6):
     """Discrete adjoint on the uniform16 recipe with the given seed: the
     production forward run, the recording run and the backward pass (HIP-event
@@ -658,6 +713,8 @@ def adjoint_measure(wr, grid=32, nsteps=20, seed=2, directions=8, delta=1e-6):
             "record_seconds": rec, "backward_seconds": bwd,
             "record_kernel_ms": rec_ms, "backward_kernel_ms": bwd_ms,
             "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd,
+            # the differentiable function's own forward pass is the recording run
+            "backward_over_record": bwd / rec,
             "cell_advances": info["cell_advances"],
             "fd_check": {"directions": directions, "delta": delta, "max_rel_err": max(errs)}}
 

@@ -314,7 +314,9 @@ int wr_last_advance_timing(const wr_context *ctx, double *total_ms, int *launche
  * density_driver; states, seeds and gradients may be host memory or device
  * memory of the context's device (as for wr_upload_cells), multipliers are
  * host memory.  e of the first state does not enter the map (eos_rt
- * recomputes it from T).  Perturbations are not part of the map. */
+ * recomputes it from T).  With wr_adjoint_set_perturb(ctx, 1) the recording
+ * applies the perturbations of a run whose loop index starts at 0 (steps 20,
+ * 40, ...; reproducer.cpp:7564-7580), and they are part of the map. */
 int wr_adjoint_ntheta(void);
 /* Run `nsteps` outer steps of the uploaded cells with multipliers M (no
  * perturbation, every cell must integrate in every step) and record the tape:
@@ -348,6 +350,12 @@ int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double 
  * budget); max_steps > 0 caps the steps per window (0: the budget alone).
  * Results do not depend on it. */
 int wr_adjoint_set_window(wr_context *ctx, int max_steps);
+/* (new) Whether subsequent wr_adjoint_record calls apply the density
+ * perturbations (0: no, the default; 1: yes, as wr_run with perturb = 1 from
+ * loop index 0).  The recording then also keeps each cell's state before every
+ * perturbation step (18 doubles per cell per 20 steps), and replay and
+ * backward differentiate through the perturbations. */
+int wr_adjoint_set_perturb(wr_context *ctx, int perturb);
 /* (new) Cells per 64-lane wave of the backward pass's reverse sweep, 1..64;
  * 0 (the default) means 64.  Results do not depend on it. */
 int wr_adjoint_set_lanes(wr_context *ctx, int cells_per_wave);

@@ -204,23 +204,86 @@ static void put_state(double *a, int n, int c, const double *st) {
     for (int f = 0; f < AD_NF; ++f) a[(size_t)f * n + c] = st[f];
 }
 
-/* Record: the gridwide loop of or_run_gridwide without perturbations, with
- * rate multipliers M, recording per step dt_grid, each cell's state at the
- * start of its advance (ckpt[step][field][cell]) and the accepted step sizes
- * (htape[cell][k], hoff[step][cell] = first index of the step).  Every cell
- * must integrate in every step.  Returns OR_SUCCESS, a failure code, or -100
- * (a cell stopped integrating) / -101 (tape capacity hcap exceeded). */
+/* Perturbation steps of a run whose loop index starts at 0
+ * (apply_perturbation, reproducer.cpp:7564-7580): step != 0, step % 20 == 0.
+ * The record keeps the state before each of them (the perturbation is not
+ * invertible: it floors and renormalises), at index step / 20 - 1. */
+static int ad_perturbed(int perturb, int step) { return perturb && step != 0 && step % 20 == 0; }
+int or_adj_npre(int nsteps) { return nsteps > 1 ? (nsteps - 1) / 20 : 0; }
+
+/* apply_perturbation on a state [18] through the generated post-step
+ * sequence (floor, normalise, charge, floor, normalise, eos_re: the same
+ * operations): xn, rho and the driver scaled by the step's factor, e kept, T
+ * recomputed; with its tangent when dst != NULL. */
+static void ad_perturb(double *st, double *dst, int cell, int step) {
+    const double f = or_perturbation_factor(cell, step);
+    double y[NEQ], ps[4] = {0.0, st[16] * f, 0.0, 0.0}, xn[NSPEC], T;
+    for (int i = 0; i < NSPEC; ++i) y[i] = st[i] * f;
+    y[NSPEC] = st[15];
+    if (dst) {
+        double dy[NEQ], pds[4] = {0.0, dst[16] * f, 0.0, 0.0}, dxn[NSPEC], dT;
+        for (int i = 0; i < NSPEC; ++i) dy[i] = dst[i] * f;
+        dy[NSPEC] = dst[15];
+        ad_post_jvp(y, ps, dy, pds, dxn, &dT);
+        memcpy(dst, dxn, sizeof(dxn));
+        dst[14] = dT;
+        dst[16] = pds[1];
+        dst[17] = dst[17] * f;
+    }
+    ad_post(y, ps, xn, &T);
+    memcpy(st, xn, sizeof(xn));
+    st[14] = T;
+    st[16] = ps[1];
+    st[17] = st[17] * f;
+}
+
+/* Adjoint of ad_perturb at the state st0 before it: lam in, the adjoint of
+ * the perturbed state; out, of st0 (T of st0 does not enter: eos_re
+ * recomputes it). */
+static void ad_perturb_adj(const double *st0, double *lam, int cell, int step) {
+    const double f = or_perturbation_factor(cell, step);
+    double y[NEQ], ps[4] = {0.0, st0[16] * f, 0.0, 0.0}, wT[1] = {lam[14]}, yb[NEQ], rb[1];
+    for (int i = 0; i < NSPEC; ++i) y[i] = st0[i] * f;
+    y[NSPEC] = st0[15];
+    ad_post_vjp(y, ps, lam, wT, yb, rb);
+    for (int i = 0; i < NSPEC; ++i) lam[i] = f * yb[i];
+    lam[14] = 0.0;
+    lam[15] = yb[NSPEC] + lam[15];
+    lam[16] = f * (rb[0] + lam[16]);
+    lam[17] = f * lam[17];
+}
+
+/* Record: the gridwide loop of or_run_gridwide (perturbations when
+ * `perturb`), with rate multipliers M, recording per step dt_grid, each
+ * cell's state at the start of its advance (ckpt[step][field][cell]), the
+ * state before each perturbation step (pre[step / 20 - 1][field][cell]) and
+ * the accepted step sizes (htape[cell][k], hoff[step][cell] = first index of
+ * the step).  Every cell must integrate in every step.  Returns OR_SUCCESS, a
+ * failure code, or -100 (a cell stopped integrating) / -101 (tape capacity
+ * hcap exceeded). */
 int or_adj_record(or_cell *cells, int n, int nsteps, const double *M, double *dts, double *ckpt,
-                  int *hoff, double *htape, int hcap) {
+                  int *hoff, double *htape, int hcap, int perturb, double *pre) {
     double grid_time = 0.0;
     int completed = 0, result = OR_SUCCESS;
     for (int c = 0; c < n; ++c) hoff[c] = 0;
     for (int step = 0; step < nsteps; ++step) {
         double dt_grid = DBL_MAX;
         int failure = OR_SUCCESS;
+        if (ad_perturbed(perturb, step))
+            for (int c = 0; c < n; ++c) {
+                const or_cell *s = &cells[c];
+                double st[AD_NF];
+                memcpy(st, s->current.xn, sizeof(double) * NSPEC);
+                st[14] = s->current.T;
+                st[15] = s->current.e;
+                st[16] = s->current.rho;
+                st[17] = s->density_driver;
+                put_state(pre + (size_t)(step / 20 - 1) * AD_NF * n, n, c, st);
+            }
         for (int c = 0; c < n; ++c) {
             int f = OR_SUCCESS;
-            const double cand = or_prepare_cell(&cells[c], c, completed, grid_time, step, 0, &f);
+            const double cand =
+                or_prepare_cell(&cells[c], c, completed, grid_time, step, perturb, &f);
             if (cand < dt_grid) dt_grid = cand;
             if (f != OR_SUCCESS && failure == OR_SUCCESS) failure = f;
         }
@@ -267,8 +330,9 @@ int or_adj_record(or_cell *cells, int n, int nsteps, const double *M, double *dt
  * (dst != NULL) or not. */
 static int ad_cell_forward(double *st, double *dst, int c, int n, int s0, int s1, const double *dts,
                            const int *hoff, const double *htape, int hcap, const double *M,
-                           const double *dM) {
+                           const double *dM, int perturb) {
     for (int step = s0; step < s1; ++step) {
+        if (ad_perturbed(perturb, step)) ad_perturb(st, dst, c, step);
         double sv[4] = {st[14], st[16], st[17], dts[step]};
         double y[NEQ], s1v[2], dy[NEQ], ds1[2] = {0.0, 0.0};
         ad_pre(st, sv, y, s1v);
@@ -313,7 +377,7 @@ static int ad_cell_forward(double *st, double *dst, int c, int n, int s0, int s1
  * with tangents ds0 -> dsN (dM: multiplier tangent) when ds0 != NULL. */
 int or_adj_replay(const double *s0, const double *ds0, int n, int nsteps, const double *dts,
                   const int *hoff, const double *htape, int hcap, const double *M,
-                  const double *dM, double *sN, double *dsN) {
+                  const double *dM, double *sN, double *dsN, int perturb) {
     int result = OR_SUCCESS;
 #pragma omp parallel for schedule(dynamic, 4) reduction(min : result)
     for (int c = 0; c < n; ++c) {
@@ -321,7 +385,7 @@ int or_adj_replay(const double *s0, const double *ds0, int n, int nsteps, const 
         get_state(s0, n, c, st);
         if (ds0) get_state(ds0, n, c, dst);
         const int r = ad_cell_forward(st, ds0 ? dst : NULL, c, n, 0, nsteps, dts, hoff, htape,
-                                      hcap, M, dM);
+                                      hcap, M, dM, perturb);
         if (r < result) result = r;
         put_state(sN, n, c, st);
         if (ds0) put_state(dsN, n, c, dst);
@@ -334,7 +398,7 @@ int or_adj_replay(const double *s0, const double *ds0, int n, int nsteps, const 
  * from its checkpoint and the substeps inside it from the stored y_k. */
 int or_adj_backward(const double *ckpt, int n, int nsteps, const double *dts, const int *hoff,
                     const double *htape, int hcap, const double *M, const double *gN, double *g0,
-                    double *Mbar) {
+                    double *Mbar, int perturb, const double *pre) {
     int result = OR_SUCCESS;
 #pragma omp parallel for schedule(dynamic, 4) reduction(min : result)
     for (int c = 0; c < n; ++c) {
@@ -377,6 +441,11 @@ int or_adj_backward(const double *ckpt, int n, int nsteps, const double *dts, co
             lam[15] = 0.0;
             lam[16] = sb[1];
             lam[17] = sb[2];
+            if (ad_perturbed(perturb, step)) {
+                double st0[AD_NF];
+                get_state(pre + (size_t)(step / 20 - 1) * AD_NF * n, n, c, st0);
+                ad_perturb_adj(st0, lam, c, step);
+            }
         }
         put_state(g0, n, c, lam);
         for (int r = 0; r < AD_NTHETA; ++r) Mbar[(size_t)r * n + c] = mb[r];

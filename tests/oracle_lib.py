@@ -288,9 +288,11 @@ class OracleAdjoint(OracleWrMath):
         sig = {
             "or_adj_ntheta": (_I, []),
             "or_lu_solve_t": (None, [_VP, _VP, _VP]),
-            "or_adj_record": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _I]),
-            "or_adj_replay": (_I, [_VP, _VP, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
-            "or_adj_backward": (_I, [_VP, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
+            "or_adj_npre": (_I, [_I]),
+            "or_adj_record": (_I, [_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _I, _I, _VP]),
+            "or_adj_replay": (_I, [_VP, _VP, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I]),
+            "or_adj_backward": (_I, [_VP, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I,
+                                     _VP]),
             "or_adj_f": (None, [_VP, _VP, _VP]),
             "or_adj_fj": (None, [_VP, _VP, _VP, _VP]),
             "or_adj_f_jvp": (None, [_VP, _VP, _VP, _VP, _VP]),
@@ -373,15 +375,19 @@ class OracleAdjoint(OracleWrMath):
                                   _p(mb))
         return mb
 
-    def record(self, cells, nsteps, M, hcap=512):
-        """Forward run with multipliers M recording the tape; returns a dict."""
+    def record(self, cells, nsteps, M, hcap=512, perturb=False):
+        """Forward run with multipliers M (and the perturbations when
+        `perturb`) recording the tape; returns a dict."""
         n = len(cells)
         M = np.ascontiguousarray(M, dtype=np.float64)
-        t = {"n": n, "nsteps": nsteps, "hcap": hcap, "M": M,
+        npre = self.lib.or_adj_npre(nsteps)
+        t = {"n": n, "nsteps": nsteps, "hcap": hcap, "M": M, "perturb": bool(perturb),
              "dts": np.zeros(nsteps), "ckpt": np.zeros((nsteps, 18, n)),
+             "pre": np.zeros((max(npre, 1), 18, n)),
              "hoff": np.zeros((nsteps + 1, n), dtype=np.int32), "htape": np.zeros((n, hcap))}
         r = self.lib.or_adj_record(_p(cells), n, nsteps, _p(M), _p(t["dts"]), _p(t["ckpt"]),
-                                   _p(t["hoff"]), _p(t["htape"]), hcap)
+                                   _p(t["hoff"]), _p(t["htape"]), hcap, int(perturb),
+                                   _p(t["pre"]))
         cells["reserved_"] = 0
         t["status"] = r
         return t
@@ -399,7 +405,8 @@ class OracleAdjoint(OracleWrMath):
         r = self.lib.or_adj_replay(_p(s0), _p(ds0) if ds0 is not None else None, n, ns,
                                    _p(t["dts"]), _p(t["hoff"]), _p(t["htape"]), t["hcap"], _p(M),
                                    _p(dM) if ds0 is not None else None, _p(sN),
-                                   _p(dsN) if ds0 is not None else None)
+                                   _p(dsN) if ds0 is not None else None,
+                                   int(t.get("perturb", False)))
         assert r == 1, r
         return (sN, dsN) if ds0 is not None else sN
 
@@ -409,6 +416,7 @@ class OracleAdjoint(OracleWrMath):
         mb = np.zeros((self.R, n))
         r = self.lib.or_adj_backward(_p(t["ckpt"]), n, ns, _p(t["dts"]), _p(t["hoff"]),
                                      _p(t["htape"]), t["hcap"], _p(t["M"]),
-                                     _p(np.ascontiguousarray(gN, dtype=np.float64)), _p(g0), _p(mb))
+                                     _p(np.ascontiguousarray(gN, dtype=np.float64)), _p(g0), _p(mb),
+                                     int(t.get("perturb", False)), _p(t["pre"]))
         assert r == 1, r
         return g0, mb

@@ -380,3 +380,74 @@ def test_runtime_reaction_table_is_the_oracle_table(adj, wr):
     derivative code was generated with (same generator, same order)."""
     assert wr.adjoint_ntheta() == adj.R
     assert wr.reaction_table() == adj.reactions()
+
+
+# ---- perturbed runs: apply_perturbation at steps 20, 40, ... is part of the map ----------
+NSTEPS_P = 45  # crosses the perturbations at steps 20 and 40
+
+
+@pytest.fixture(scope="module")
+def recorded_perturbed(adj, wr):
+    n = 8
+    seed, sigma, t0, t1 = wr.RECIPES["uniform16"]
+    cells = adj.make_recipe_cells(n, wr.CELL_DTYPE, 2, sigma, t0, t1)
+    plain = cells.copy()
+    tape = adj.record(cells, NSTEPS_P, np.ones(adj.R), perturb=True)
+    assert tape["status"] == 1
+    return cells, plain, tape
+
+
+def test_perturbed_record_is_the_plain_perturbed_run(adj, wr, recorded_perturbed):
+    cells, plain, tape = recorded_perturbed
+    r, completed = adj.run_gridwide(plain, True, max_steps=NSTEPS_P)
+    assert r == 1 and completed == NSTEPS_P
+    mask = np.ones(216, bool)
+    mask[156:160] = False
+    n = len(cells)
+    assert np.array_equal(cells.view(np.uint8).reshape(n, 216)[:, mask],
+                          plain.view(np.uint8).reshape(n, 216)[:, mask])
+    # the state before each perturbation is the state after the step before it
+    assert not np.array_equal(tape["pre"][0], tape["ckpt"][20])
+
+
+def test_perturbed_replay_reproduces_the_record(adj, recorded_perturbed):
+    cells, _, tape = recorded_perturbed
+    sN = adj.replay(tape, tape["ckpt"][0])
+    assert np.array_equal(sN.view(np.uint64), _state_of(cells).view(np.uint64))
+
+
+def test_perturbed_tangent_matches_central_differences(adj, recorded_perturbed):
+    _, _, tape = recorded_perturbed
+    s0 = tape["ckpt"][0]
+    M = tape["M"]
+    rng = np.random.default_rng(21)
+    delta = 1e-6
+    for k in range(6):
+        v, dth = _direction(rng, s0, adj.R)
+        dM = dth * M
+        _, tl = adj.replay(tape, s0, ds0=v, dM=dM)
+        sp = adj.replay(tape, s0 + delta * v, M=M * np.exp(delta * dth))
+        sm = adj.replay(tape, s0 - delta * v, M=M * np.exp(-delta * dth))
+        fd = (sp - sm) / (2 * delta)
+        for rows in (slice(0, 14), slice(14, 15), slice(15, 16), slice(16, 18)):
+            scale = np.max(np.abs(fd[rows])) + 1e-300
+            assert np.max(np.abs(fd[rows] - tl[rows])) <= 1e-4 * scale, (k, rows)
+
+
+def test_perturbed_adjoint_is_the_transposed_tangent(adj, recorded_perturbed):
+    _, _, tape = recorded_perturbed
+    s0 = tape["ckpt"][0]
+    rng = np.random.default_rng(22)
+    for k in range(4):
+        gN = rng.standard_normal(s0.shape)
+        g0, mb = adj.backward(tape, gN)
+        v, dM = _direction(rng, s0, adj.R)
+        _, tl = adj.replay(tape, s0, ds0=v, dM=dM)
+        lhs = np.sum(gN * tl)
+        rhs = np.sum(g0 * v) + mb.sum(axis=1) @ dM
+        assert abs(lhs - rhs) <= 1e-11 * (np.sum(np.abs(gN * tl)) + abs(lhs)), (lhs, rhs)
+        assert np.all(g0[15] == 0.0)
+    # the perturbation enters the gradient: without it the same seed differs
+    flat = dict(tape, perturb=False)
+    g0f, _ = adj.backward(flat, gN)
+    assert not np.array_equal(g0, g0f)

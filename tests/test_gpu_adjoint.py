@@ -106,6 +106,35 @@ def test_backward_matches_oracle(recorded, adj):
     assert np.array_equal(_bits(mb), _bits(omb))
 
 
+def test_perturbed_record_replay_backward_match_oracle(wr, adj):
+    """A recording with the density perturbations of steps 20 and 40 (45
+    steps, 4^3 cells; wr_adjoint_set_perturb): final cells, the frozen-map
+    replay and the backward pass equal the oracle's (whose perturbed tangent
+    and adjoint are checked against finite differences on the CPU) bit for bit."""
+    n, nsteps = 4 ** 3, 45
+    print(f"perturbed adjoint: {n} cells, {nsteps} steps", flush=True)
+    cells, ocells = _cells(wr, adj, n)
+    with wr.Context(n) as ctx:
+        ctx.upload(cells)
+        done = ctx.adjoint_record(nsteps, np.ones(wr.adjoint_ntheta()), hcap=1024, perturb=True)
+        assert done == nsteps
+        ctx.download(cells)
+        tape = adj.record(ocells, nsteps, np.ones(adj.R), hcap=1024, perturb=True)
+        assert tape["status"] == 1
+        assert _cells_equal(cells, ocells)
+        sN = ctx.adjoint_replay()
+        assert np.array_equal(_bits(sN), _bits(adj.replay(tape, tape["ckpt"][0])))
+        rng = np.random.default_rng(31)
+        gN = rng.standard_normal((18, n))
+        g0, mb = ctx.adjoint_backward(gN)
+        og0, omb = adj.backward(tape, gN)
+        assert np.array_equal(_bits(g0), _bits(og0))
+        assert np.array_equal(_bits(mb), _bits(omb))
+        # the next recording on the context is unperturbed again
+        ctx.upload(wr.make_recipe_states(n, 2, *wr.RECIPES["uniform16"][1:]))
+        ctx.adjoint_record(5, np.ones(wr.adjoint_ntheta()))
+
+
 def test_autograd_function_gradients(wr, adj):
     """torch.autograd through the GPU adjoint: the loss sum(H2 mass fraction +
     log T) of 4^3 cells over 20 steps; theta and s0 gradients against central

@@ -7,7 +7,8 @@
             reaction_table() names them)
     s0    : (18, n) state of each cell at its first advance, field-major:
             xn[0..13] (number densities), T, e, rho, density_driver
-    ->  sN: (18, n) state after `nsteps` outer steps (no perturbations)
+    ->  sN: (18, n) state after `nsteps` outer steps (with the density
+            perturbations at steps 20, 40, ... when perturb=True)
 
 The forward pass is a recording run on the GPU (the reference integrator on
 f_theta, wr_adjoint_record): it checkpoints every outer step and stores the
@@ -98,7 +99,7 @@ class CollapseAdjoint(torch.autograd.Function):
     """sN = collapse(theta, s0) on a wr.Context; gradients by the discrete adjoint."""
 
     @staticmethod
-    def forward(fctx, theta, s0, context, nsteps, hcap):
+    def forward(fctx, theta, s0, context, nsteps, hcap, perturb=False):
         if theta.shape != (adjoint_ntheta(),):
             raise ValueError(f"theta must have shape ({adjoint_ntheta()},)")
         M = torch.exp(theta.detach().to(torch.float64)).cpu().numpy()
@@ -110,13 +111,13 @@ class CollapseAdjoint(torch.autograd.Function):
             cells = cells_tensor_from_state(s0.detach().to(torch.float64)).contiguous()
             torch.cuda.synchronize(s0.device)  # the records are complete before the copy
             context.upload_ptr(cells.data_ptr(), cells.shape[0])
-            context.adjoint_record(nsteps, M, hcap)
+            context.adjoint_record(nsteps, M, hcap, perturb)
             context.download_ptr(cells.data_ptr(), cells.shape[0])
             fctx.generation = context.adjoint_generation()  # the tape this forward recorded
             return state_of_cells_tensor(cells)
         cells = cells_from_state(s0.detach().to(torch.float64).numpy())
         context.upload(cells)
-        context.adjoint_record(nsteps, M, hcap)
+        context.adjoint_record(nsteps, M, hcap, perturb)
         context.download(cells)
         fctx.generation = context.adjoint_generation()
         return torch.from_numpy(state_of(cells))
@@ -135,17 +136,17 @@ class CollapseAdjoint(torch.autograd.Function):
                                               fctx.generation)
             M = torch.from_numpy(fctx.M).to(device=g.device)
             gtheta = mbar.sum(dim=1) * M  # dM/dtheta = M
-            return gtheta.to(fctx.theta_device), g0, None, None, None
+            return gtheta.to(fctx.theta_device), g0, None, None, None, None
         g0, mbar = fctx.context.adjoint_backward(gN.detach().to(torch.float64).numpy(),
                                                  generation=fctx.generation)
         gtheta = mbar.sum(axis=1) * fctx.M  # dM/dtheta = M
         return (torch.from_numpy(gtheta).to(fctx.theta_device), torch.from_numpy(g0), None, None,
-                None)
+                None, None)
 
 
-def collapse(theta, s0, context, nsteps=20, hcap=512):
+def collapse(theta, s0, context, nsteps=20, hcap=512, perturb=False):
     """Differentiable collapse advance of the cells in s0 (see module doc)."""
-    return CollapseAdjoint.apply(theta, s0, context, nsteps, hcap)
+    return CollapseAdjoint.apply(theta, s0, context, nsteps, hcap, perturb)
 
 
 H2 = 8  # species index of H2 (reproducer.cpp:228-229)

@@ -6,7 +6,8 @@
 // of a recorded run held fixed -- dt_grid of each step and the step sizes of
 // the accepted ROS2S attempts (rejected attempts leave y unchanged).  Per
 // step a cell maps (xn, T, rho, driver) -pre-> (y0, rho1, driver1) -accepted
-// ROS2S steps-> y_n -post-> (xn', T'), with e' = y_n[14].  Parameters: the
+// ROS2S steps-> y_n -post-> (xn', T'), with e' = y_n[14]; in a perturbed
+// recording the perturbation of steps 20, 40, ... comes first.  Parameters: the
 // state at the first advance and the rate multipliers M_r = exp(theta_r) of
 // the network's 50 reaction flux terms (the generated f_theta).
 //
@@ -72,9 +73,68 @@ struct AdjDev {
     const double* M; // [R]
     double* ws;      // [kWsFixed][cap]
     int* err;        // first error: -100 a cell did not integrate, -101 tape full
+    const double* pre;  // [npre][18][cap]: state before each perturbation step
     int hcap, nsteps, cap;
     bool unit;       // every multiplier of M is exactly 1
+    bool perturb;    // the recording applied the perturbations (steps 20, 40, ...)
 };
+
+// apply_perturbation (reproducer.cpp:7564-7580) of the run whose loop index
+// starts at 0: steps 20, 40, ...; the record keeps the state before each at
+// index step / 20 - 1 (the map floors and renormalises: it cannot be undone).
+__device__ __forceinline__ bool adj_perturbed(const AdjDev& a, int step) {
+    return a.perturb && step != 0 && step % 20 == 0;
+}
+
+// The perturbation on a state [18] through the generated post-step sequence
+// (the operations of post_step): xn, rho and driver scaled by the step's
+// factor, e kept, T recomputed.
+__device__ __forceinline__ void adj_perturb(double (&sv)[18], int c, int step) {
+    const double f = perturbation_factor(c, step);
+    double y[kNeqs], xn[kNumSpec], T;
+    const double ps[4] = {0.0, sv[16] * f, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < kNumSpec; ++i) y[i] = sv[i] * f;
+    y[kNumSpec] = sv[15];
+    ad_post(y, ps, xn, &T);
+#pragma unroll
+    for (int i = 0; i < kNumSpec; ++i) sv[i] = xn[i];
+    sv[14] = T;
+    sv[16] = ps[1];
+    sv[17] = sv[17] * f;
+}
+
+// Its adjoint at the state st0 before it: lam in, the adjoint of the
+// perturbed state; out, of st0 (T of st0 does not enter).
+__device__ __forceinline__ void adj_perturb_adj(const double (&st0)[18], double (&lam)[18], int c,
+                                                int step) {
+    const double f = perturbation_factor(c, step);
+    double y[kNeqs], yb[kNeqs], rb[1];
+    const double ps[4] = {0.0, st0[16] * f, 0.0, 0.0};
+    const double wT[1] = {lam[14]};
+#pragma unroll
+    for (int i = 0; i < kNumSpec; ++i) y[i] = st0[i] * f;
+    y[kNumSpec] = st0[15];
+    ad_post_vjp(y, ps, lam, wT, yb, rb);
+#pragma unroll
+    for (int i = 0; i < kNumSpec; ++i) lam[i] = f * yb[i];
+    lam[14] = 0.0;
+    lam[15] = yb[kNumSpec] + lam[15];
+    lam[16] = f * (rb[0] + lam[16]);
+    lam[17] = f * lam[17];
+}
+
+// The cells' state before a perturbation step, into pre [18][cap].
+__global__ void k_adj_pre_ckpt(DevCells s, double* __restrict__ pre) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= s.n) return;
+#pragma unroll
+    for (int k = 0; k < kNumSpec; ++k) pre[k * s.cap + c] = s.xn[k * s.cap + c];
+    pre[14 * s.cap + c] = s.T[c];
+    pre[15 * s.cap + c] = s.e[c];
+    pre[16 * s.cap + c] = s.rho[c];
+    pre[17 * s.cap + c] = s.driver[c];
+}
 
 __device__ __forceinline__ void adj_error(int* err, int code) { atomicCAS(err, 0, code); }
 
@@ -414,6 +474,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_replay(AdjDev a, const double
 #pragma unroll
     for (int f = 0; f < kNF; ++f) sv[f] = s0[static_cast<size_t>(f) * a.cap + c];
     for (int step = 0; step < a.nsteps; ++step) {
+        if (adj_perturbed(a, step)) adj_perturb(sv, c, step);
         const double pin[4] = {sv[14], sv[16], sv[17], a.dts[step]};
         double y[kNeqs], s1[2];
         ad_pre(sv, pin, y, s1);
@@ -553,6 +614,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const doubl
         lam[15] = 0.0;
         lam[16] = sb[1];
         lam[17] = sb[2];
+        if (adj_perturbed(a, step)) {
+            double st0[kNF];
+            const double* pr = a.pre + static_cast<size_t>(step / 20 - 1) * kNF * a.cap + c;
+#pragma unroll
+            for (int f = 0; f < kNF; ++f) st0[f] = pr[f * a.cap];
+            adj_perturb_adj(st0, lam, c, step);
+        }
     }
 #pragma unroll
     for (int f = 0; f < kNF; ++f) lamv[static_cast<size_t>(f) * a.cap + c] = lam[f];
@@ -571,6 +639,9 @@ struct AdjointState {
     int window_max = 0;       // most steps per backward window (0: by memory budget)
     int lanes_per_wave = 0;   // cells per wave of the reverse sweep (0: 64)
     std::vector<int> hoff_h;  // host copy of the tape index [nsteps + 1][cap]
+    double* pre = nullptr;    // [npre][18][cap]: state before each perturbation step
+    size_t pre_elems = 0;
+    bool perturb = false;     // the recording applied the perturbations
     double* sub = nullptr;    // stored substeps of one window of steps
     size_t sub_elems = 0;
     double* sub2 = nullptr;   // the other buffer: the next window's substeps are
@@ -613,6 +684,7 @@ void wr::detail::adjoint_destroy(wr_context* ctx) {
     (void)hipFree(a->ws);
     (void)hipFree(a->sub);
     (void)hipFree(a->sub2);
+    (void)hipFree(a->pre);
     for (int b = 0; b < 2; ++b) {
         if (a->ready[b]) (void)hipEventDestroy(a->ready[b]);
         if (a->freed[b]) (void)hipEventDestroy(a->freed[b]);
@@ -638,6 +710,8 @@ AdjDev dev_view(const AdjointState* a, const double* M) {
     d.M = M;
     d.ws = a->ws;
     d.err = a->err;
+    d.pre = a->pre;
+    d.perturb = a->perturb;
     d.hcap = a->hcap;
     d.nsteps = a->nsteps;
     d.cap = a->cap;
@@ -730,8 +804,14 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     }
     a->nsteps = nsteps;
     a->n = ctx->n;
+    a->perturb = ctx->adj_perturb != 0;
     int st = ensure_ws(ctx, a);
     if (st != WR_SUCCESS) return st;
+    const int npre = (a->perturb && nsteps > 1) ? (nsteps - 1) / 20 : 0;
+    if (npre > 0 &&
+        (st = ensure_buf(&a->pre, &a->pre_elems, static_cast<size_t>(npre) * kNF * a->cap,
+                         "hipMalloc(adjoint pre-perturbation states)")) != WR_SUCCESS)
+        return st;
     WR_HIP(hipMemcpyAsync(a->M, multipliers, kR * sizeof(double), hipMemcpyHostToDevice,
                           ctx->stream),
            "hipMemcpy(multipliers)");
@@ -740,7 +820,7 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     WR_HIP(hipMemsetAsync(a->err, 0, sizeof(int), ctx->stream), "hipMemset");
     WR_HIP(hipMemsetAsync(a->hoff, 0, static_cast<size_t>(a->cap) * sizeof(int), ctx->stream),
            "hipMemset");
-    if ((st = ctl_init(ctx, 0, nsteps, 0, 0.0)) != WR_SUCCESS) return st;
+    if ((st = ctl_init(ctx, a->perturb ? 1 : 0, nsteps, 0, 0.0)) != WR_SUCCESS) return st;
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
     // the production queue-fed kernel on f_theta, with the tape hooks
     int cus = 0, per_cu = 0;
@@ -754,6 +834,12 @@ int wr_adjoint_record(wr_context* ctx, int nsteps, const double* multipliers, in
     if (qblocks > blocks_for(ctx->cap, ctx->q_threads)) qblocks = blocks_for(ctx->cap, ctx->q_threads);
     if (qblocks < 1) qblocks = 1;
     for (int step = 0; step < nsteps; ++step) {
+        if (a->perturb && step != 0 && step % 20 == 0) {  // the state the perturbation starts from
+            hipLaunchKernelGGL(k_adj_pre_ckpt, dim3(blocks_for(ctx->n, 256)), dim3(256), 0,
+                               ctx->stream, ctx->cells,
+                               a->pre + static_cast<size_t>(step / 20 - 1) * kNF * a->cap);
+            WR_HIP(hipGetLastError(), "k_adj_pre_ckpt launch");
+        }
         if ((st = ctl_prepare(ctx)) != WR_SUCCESS) return st;
         if ((st = ctl_queue_order(ctx)) != WR_SUCCESS) return st;
         QThetaNet net{a->M, a->ckpt, a->hoff, a->htape, a->err, a->hcap, a->cap, step, 0, a->unit};
@@ -982,6 +1068,13 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     int code = 0;
     if ((st = read_err(ctx, a, &code)) != WR_SUCCESS) return st;
     return code == 0 ? kSuccess : kLuError;
+}
+
+int wr_adjoint_set_perturb(wr_context* ctx, int perturb) {
+    if (!ctx || perturb < 0 || perturb > 1)
+        return fail(WR_ERR_ARGUMENT, "wr_adjoint_set_perturb: perturb must be 0 or 1");
+    ctx->adj_perturb = perturb;
+    return WR_SUCCESS;
 }
 
 int wr_adjoint_set_window(wr_context* ctx, int max_steps) {

@@ -164,6 +164,7 @@ struct wr_context {
     // or WR_POLICY_GRIDWIDE_CTA32
     int policy = 0;
     int cta_blocks = 0;  // persistent grid of the gridwide-cta32 kernel (0: not sized yet)
+    int adj_perturb = 0;  // subsequent adjoint recordings apply the perturbations
     size_t cell_bytes = 0;           // bytes of cells_mem
     void* local_backup = nullptr;    // cells at the start of a legacy-local step
     int q_blocks_local = 0;          // persistent grid of the legacy-local kernel

@@ -109,6 +109,7 @@ _SIGS = {
     "wr_adjoint_backward": (_I, [_VP, _VP, _VP, _VP]),
     "wr_adjoint_generation": (_I, [_VP, ctypes.POINTER(ctypes.c_ulonglong)]),
     "wr_adjoint_backward_gen": (_I, [_VP, ctypes.c_ulonglong, _VP, _VP, _VP]),
+    "wr_adjoint_set_perturb": (_I, [_VP, _I]),
     "wr_adjoint_tape": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "wr_adjoint_last_ms": (_I, [_VP, ctypes.POINTER(_D)]),
     "wr_adjoint_set_window": (_I, [_VP, _I]),
@@ -436,9 +437,12 @@ class Context:
         return x
 
     # ---- discrete adjoint (frozen controller decisions of a recorded run) ----
-    def adjoint_record(self, nsteps, multipliers, hcap=512):
-        """Run nsteps outer steps of the uploaded cells with rate multipliers,
+    def adjoint_record(self, nsteps, multipliers, hcap=512, perturb=False):
+        """Run nsteps outer steps of the uploaded cells with rate multipliers
+        (and the density perturbations at steps 20, 40, ... when `perturb`),
         recording the tape; returns the completed step count."""
+        _check(self._lib.wr_adjoint_set_perturb(self._h, int(bool(perturb))),
+               "wr_adjoint_set_perturb")
         M = np.ascontiguousarray(multipliers, dtype=np.float64)
         done = ctypes.c_int()
         r = self._lib.wr_adjoint_record(self._h, int(nsteps), _ptr(M), int(hcap), ctypes.byref(done))
