@@ -92,6 +92,10 @@ enum { OR_SUCCESS = 1, OR_BAD_INPUTS = -1, OR_DT_UNDERFLOW = -2, OR_TOO_MANY_STE
 #define CHEM_OUT_J(r, c, v) (jout[(r) * NEQ + (c)] = (v))
 #define CHEM_OUT_F(i, v) (fout[(i)] = (v))
 #include "_build/chem_network.h"
+#ifdef OR_RATE_TABLE
+#include "_build/chem_table.h"
+#include "rate_table.c"
+#endif
 
 /* ---- network constants and EOS (reproducer.cpp:199-339) ----------------- */
 static const double k_mass[NSPEC] = {
@@ -188,6 +192,14 @@ static double temperature_from_y(const double *y, double *X) {
 void or_rhs(const double *y, double *dydt) {
     double X[NSPEC];
     const double T = temperature_from_y(y, X);
+#ifdef OR_RATE_TABLE
+    if (g_rt_mode) {
+        double tv[CHEM_NTV];
+        rt_values(T, tv);
+        chem_rhs_tv(tv, T, X, dydt);
+        return;
+    }
+#endif
     chem_rhs(T, X, dydt);
 }
 
@@ -195,6 +207,14 @@ void or_rhs(const double *y, double *dydt) {
 void or_jac(const double *y, double *jac) {
     double X[NSPEC], f[NEQ];
     const double T = temperature_from_y(y, X);
+#ifdef OR_RATE_TABLE
+    if (g_rt_mode) {
+        double tv[CHEM_NTV];
+        rt_values(T, tv);
+        chem_fused_tv(tv, T, X, f, jac);
+        return;
+    }
+#endif
     chem_fused(T, X, f, jac);
 }
 
@@ -202,6 +222,14 @@ void or_jac(const double *y, double *jac) {
 void or_jac_rhs(const double *y, double *jac, double *dydt) {
     double X[NSPEC];
     const double T = temperature_from_y(y, X);
+#ifdef OR_RATE_TABLE
+    if (g_rt_mode) {
+        double tv[CHEM_NTV];
+        rt_values(T, tv);
+        chem_fused_tv(tv, T, X, dydt, jac);
+        return;
+    }
+#endif
     chem_fused(T, X, dydt, jac);
 }
 

@@ -10,6 +10,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
 ORACLE_WR_SO = os.path.join(REPO, "oracle", "_build", "liboracle_wrmath.so")
+ORACLE_TABLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle_table.so")
 REF_SO = os.path.join(REPO, "oracle", "_ref", "libwr_ref.so")
 REF_CLI = os.path.join(REPO, "oracle", "_ref", "reproducer_cpu")
 
@@ -253,6 +254,40 @@ class OracleWrMath(_Cpu):
     def libm(self, op, x):
         fn = {"exp": self._libm_exp, "log": self._libm_log, "cbrt": self._libm_cbrt}[op]
         return np.array([fn(float(v)) for v in np.ravel(x)])
+
+
+class OracleTable(_Cpu):
+    """The oracle with the rate-table study compiled in (oracle/rate_table.c):
+    mode 0 the network as generated, 1 the network split at its T-only
+    frontier with exact values, 2 the frontier values from a cubic table in
+    ln T.  The mode is process-wide; reset it to 0 after use."""
+    prefix = "or_"
+
+    def __init__(self):
+        if not os.path.exists(ORACLE_TABLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_TABLE_SO} (make -C oracle)")
+        super().__init__(ORACLE_TABLE_SO)
+        self.lib.or_rt_set.restype = _I
+        self.lib.or_rt_set.argtypes = [_I, _D, _D, _I]
+        self.lib.or_rt_ntv.restype = _I
+        self.lib.or_rt_outside.restype = ctypes.c_longlong
+        self.lib.or_rt_errors.argtypes = [_VP, _I, _VP]
+        self.ntv = self.lib.or_rt_ntv()
+
+    def rt_set(self, mode, tlo=1.0, thi=2.0, n=4):
+        if self.lib.or_rt_set(mode, tlo, thi, n) != 0:
+            raise ValueError(f"or_rt_set({mode}, {tlo}, {thi}, {n}) refused")
+
+    def rt_outside(self):
+        """Frontier evaluations outside the table's range since rt_set."""
+        return self.lib.or_rt_outside()
+
+    def rt_errors(self, T):
+        """Per frontier value, the max relative interpolation error at T."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        err = np.zeros(self.ntv)
+        self.lib.or_rt_errors(_p(T), len(T), _p(err))
+        return err
 
 
 class Reference(_Cpu):
