@@ -1,5 +1,5 @@
 """The network generator as a tool (gen/dag.py, gen/import_dag.py, gen/emit.py,
-gen/stage.py, gen/coop.py), restating the reference's own slicer tests
+gen/stage.py, gen/coop.py, gen/table.py), restating the reference's own slicer tests
 (tests/test_slice_dag.py:27-78 parser cases, 82-161 manifest and deterministic
 output) for this build's parser and emitters:
 
@@ -88,6 +88,7 @@ def test_reimport_reproduces_the_committed_ir(tmp_path):
 @pytest.mark.parametrize("script,outputs", [
     ("emit.py", ["chem_network.h", "chem_counts.json"]),
     ("coop.py", ["coop_network.h", "coop_stats.json"]),
+    ("table.py", ["chem_table.h", "chem_table.json"]),
 ])
 def test_emitters_are_deterministic(tmp_path, script, outputs):
     digests = []
@@ -110,3 +111,29 @@ def test_emitted_op_counts_are_pinned(tmp_path):
     assert sum(c["chem_rhs"].get(op, 0) for op in flops) == 1536
     assert c["chem_fused"]["exp"] == 278 and c["chem_fused"]["log"] == 69
     assert c["staged"]["fused"]["R"] == 128
+
+
+def test_table_frontier_is_smooth_and_temperature_only():
+    """The rate-table split (gen/table.py): every frontier value depends on T
+    and constants only, through no selection or test, and everything the
+    network consumes from the T-only part that is not a frontier value is a
+    selection, a test or a leaf (so the split network computes it from T)."""
+    import emit
+    import table
+    d = emit.specialise(dagmod.read_ir(IR))
+    roots = list(d.outputs.values())
+    fr = table.frontier(d, roots)
+    sm = table.smooth(d)
+    assert len(fr) == 441 and len(set(fr)) == len(fr)
+    for nid in fr:
+        assert sm[nid] and d.nodes[nid].op not in table.LEAF_OPS
+    live = table.live_with_leaves(d, roots, set(fr))
+    for nid in live:
+        n = d.nodes[nid]
+        if sm[nid] and nid not in fr:
+            assert n.op in ("const", "T"), n.op  # smooth values only enter through tv
+    ops = {}
+    for nid in d.live_nodes(fr):
+        op = d.nodes[nid].op
+        ops[op] = ops.get(op, 0) + 1
+    assert ops.get("exp") == 213 and ops.get("log") == 24 and "sel" not in ops
