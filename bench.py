@@ -23,7 +23,9 @@ same workload and launch shape, each calibrated by 1 GiB device copies in the
 same process (warp-reactor_amd/prof/traffic_run.py).
 `plain_fp64` times the same run with the thread-per-cell kernel (no queue,
 network left to the compiler's register allocation) and checks that the
-tuned default produces bit-identical final states.
+tuned default produces bit-identical final states.  `adjoint_default_run`
+differentiates the first 100 steps of the same run; it is measured in a child
+process started before this one uses the GPU.
 """
 import argparse
 import importlib
@@ -235,7 +237,8 @@ def main():
     ap.add_argument("--no-cta32", action="store_true",
                     help="skip the gridwide-cta32 policy run")
     ap.add_argument("--no-adjoint", action="store_true",
-                    help="skip the adjoint measurement (32^3, 20 outer steps)")
+                    help="skip the adjoint measurements (32^3, 20 outer steps; and the "
+                         "default run's first 100 steps in a child process)")
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue outer steps as individual launches instead of HIP graphs")
     ap.add_argument("--no-autotune", action="store_true",
@@ -255,6 +258,9 @@ def main():
     if args.adjoint_default_run:
         print(json.dumps({"adjoint_default_run": adjoint_default_run_measure(wr)}), flush=True)
         return
+    adjoint_default_run = None
+    if not args.no_adjoint and args.grid == 64 and args.recipe == "reference":
+        adjoint_default_run = adjoint_default_run_child()
     n = args.grid ** 3
     if args.recipe == "reference":
         init = wr.make_collapse_states(n)
@@ -395,6 +401,8 @@ def main():
         line["lognormal128"] = lognormal128_measure(wr, ctx_shape)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_measure(wr)
+        if adjoint_default_run is not None:
+            line["adjoint_default_run"] = adjoint_default_run
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(init, dts, args.cpu_budget)
     kname = line["roofline"]["kernel"]
@@ -599,14 +607,31 @@ def cta32_measure(ctx, init, max_outer, ref_cells, n, wr):
                 "max_thermodynamic_rel_error", "max_deuterium_species_rel_error")}}
 
 
+def adjoint_default_run_child(timeout=600):
+    """adjoint_default_run_measure in a process of its own (this script with
+    --adjoint-default-run), started before this process uses the GPU; its
+    result, or the error it ended with."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--adjoint-default-run"]
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout} s"}
+    sys.stderr.write("".join(l + "\n" for l in p.stderr.splitlines() if l.startswith("#")))
+    sys.stderr.flush()
+    if p.returncode != 0:
+        return {"error": f"exit status {p.returncode}", "stderr_tail": p.stderr[-600:]}
+    return json.loads(p.stdout.strip().splitlines()[-1])["adjoint_default_run"]
+
+
 def adjoint_default_run_measure(wr, grid=64, nsteps=100, hcap=4096, directions=2, delta=1e-6):
     """The discrete adjoint of the first `nsteps` outer steps of the default
     run (reference initial state, density perturbations at steps 20, 40, ...):
     forward, record and backward times, and the gradient of sum(log T) against
     central differences of the frozen-map replay along seeded directions.
     Runs only under --adjoint-default-run, in a process that has run nothing
-    else: after the other measurements (torch's autograd among them) the first
-    64^3 reverse dispatch of this run was refused its scratch memory
+    else (the default bench starts it as a child first): run after the other
+    measurements in one process (torch's autograd among them), the first
+    64^3 reverse dispatch was refused its scratch memory
     (HSA_STATUS_ERROR_OUT_OF_RESOURCES)."""
     n = grid ** 3
     init = wr.make_collapse_states(n)
