@@ -502,15 +502,19 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_replay(AdjDev a, const double
 }
 
 // ---- the adjoint -------------------------------------------------------------------------
-// Phase A, one thread per (cell, step) of a window of steps [s0, s1): recompute the
-// step from its checkpoint and store each accepted substep's pieces in slot
-// j = k - hoff[s0] of the cell (sub[(j * kSub + field) * cap + cell]).  The
-// thread's workspace column (same layout as the per-cell one) has stride nA.
-__global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __restrict__ sub, int n,
+// Phase A, one thread per (position, step) of a window of steps [s0, s1): the
+// cell at position q of the window's order (perm[q], the cells by their
+// substeps in the window, most first) recomputes the step from its checkpoint
+// and stores each accepted substep's pieces in slot j = k - hoff[s0] of column
+// q (sub[(j * kSub + field) * cap + q]).  The thread's workspace column (same
+// layout as the per-cell one) has stride nA.
+__global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __restrict__ sub,
+                                                         const int* __restrict__ perm, int n,
                                                          int s0, int s1, int zero) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * (s1 - s0)) return;
-    const int c = t % n;
+    const int q = t % n;
+    const int c = perm[q];
     const int step = s0 + t / n;
     // the step's workspace in lane-private memory (`zero` is always 0)
     double priv[kWsSteps];
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __res
     const int k0 = a.hoff[static_cast<size_t>(step) * a.cap + c];
     const int k1 = a.hoff[static_cast<size_t>(step + 1) * a.cap + c];
     for (int k = k0; k < k1; ++k) {
-        double* slot = sub + static_cast<size_t>(k - base) * kSub * a.cap + c;
+        double* slot = sub + static_cast<size_t>(k - base) * kSub * a.cap + q;
 #pragma unroll
         for (int i = 0; i < kNeqs; ++i) slot[(kSY + i) * a.cap] = y[i];
         uint64_t piv;
@@ -547,18 +551,22 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_steps(AdjDev a, double* __res
     }
 }
 
-// Phase B, one thread per cell: the window's steps in reverse from the stored
-// substeps.  lam [18][cap] carries dL/d(state) across windows; dL/dM
-// accumulates in the workspace (cleared when `first`).  Each 64-lane wave
-// (one block) takes `lpw` cells on its first lanes (64 by default; fewer put
-// waves on more SIMDs of a small grid).
+// Phase B, one thread per position of the window's order (the same perm as
+// phase A, so the heaviest cells start first and a wave's cells cost alike):
+// the window's steps in reverse from the stored substeps in column q.  lam
+// [18][cap] carries dL/d(state) across windows; dL/dM accumulates in the
+// workspace (cleared when `first`); both are per cell.  Each 64-lane wave
+// (one block) takes `lpw` positions on its first lanes (64 by default; fewer
+// put waves on more SIMDs of a small grid).
 __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const double* __restrict__ sub,
+                                                           const int* __restrict__ perm,
                                                            double* __restrict__ lamv, int n, int s0,
                                                            int s1, int first, int zero, int lpw) {
     using C = Ros2sCoef;
     if (static_cast<int>(threadIdx.x) >= lpw) return;
-    const int c = blockIdx.x * lpw + threadIdx.x;
-    if (c >= n) return;
+    const int q = blockIdx.x * lpw + threadIdx.x;
+    if (q >= n) return;
+    const int c = perm[q];
     double* w = a.ws + c;
     const int st = a.cap;
     // `zero` (always 0) keeps the array in private memory rather than registers
@@ -581,7 +589,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const doubl
         const int k1 = a.hoff[static_cast<size_t>(step + 1) * a.cap + c];
         if (k1 > k0) {
             // y after the step's last substep, from its stored pieces
-            const double* slot = sub + static_cast<size_t>(k1 - 1 - base) * kSub * a.cap + c;
+            const double* slot = sub + static_cast<size_t>(k1 - 1 - base) * kSub * a.cap + q;
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i)
                 y[i] = slot[(kSY + i) * a.cap] + C::b1 * slot[(kSK1 + i) * a.cap] +
@@ -595,7 +603,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_reverse(AdjDev a, const doubl
         yb[kNumSpec] += lam[15];
         const double rho1b = lam[16] + rb[0];
         for (int k = k1 - 1; k >= k0; --k) {
-            const double* slot = sub + static_cast<size_t>(k - base) * kSub * a.cap + c;
+            const double* slot = sub + static_cast<size_t>(k - base) * kSub * a.cap + q;
             double yk[kNeqs];
 #pragma unroll
             for (int i = 0; i < kNeqs; ++i) yk[i] = slot[(kSY + i) * a.cap];
@@ -646,6 +654,9 @@ struct AdjointState {
     size_t sub_elems = 0;
     double* sub2 = nullptr;   // the other buffer: the next window's substeps are
     size_t sub2_elems = 0;    // recomputed while the current window is reversed
+    int* perm = nullptr;      // [windows][cap]: each window's cells by substeps, most first
+    size_t perm_elems = 0;
+    std::vector<int> perm_h;  // host side of perm (kept until the next backward call)
     hipStream_t steps_stream = nullptr;
     hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, go = nullptr;
     bool recorded = false;
@@ -685,6 +696,7 @@ void wr::detail::adjoint_destroy(wr_context* ctx) {
     (void)hipFree(a->sub);
     (void)hipFree(a->sub2);
     (void)hipFree(a->pre);
+    (void)hipFree(a->perm);
     for (int b = 0; b < 2; ++b) {
         if (a->ready[b]) (void)hipEventDestroy(a->ready[b]);
         if (a->freed[b]) (void)hipEventDestroy(a->freed[b]);
@@ -1039,7 +1051,38 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     }
     double* const buf[2] = {a->sub, a->sub2};
     if ((st = timed_begin(ctx, a)) != WR_SUCCESS) return st;
-    // the recompute starts after the status word is cleared
+    // each window's order: its cells by substeps in the window, most first,
+    // ties by cell index (a counting sort over the host copy of the tape index)
+    a->perm_h.resize(wins.size() * cap);
+    {
+        std::vector<int> count;
+        for (size_t i = 0; i < wins.size(); ++i) {
+            const Window& wdw = wins[i];
+            const int* h0 = a->hoff_h.data() + static_cast<size_t>(wdw.s0) * cap;
+            const int* h1 = a->hoff_h.data() + static_cast<size_t>(wdw.s1) * cap;
+            count.assign(static_cast<size_t>(wdw.most) + 2, 0);
+            for (int c = 0; c < n; ++c) ++count[wdw.most - (h1[c] - h0[c])];
+            int at = 0;
+            for (auto& v : count) {
+                const int m = v;
+                v = at;
+                at += m;
+            }
+            int* out = a->perm_h.data() + i * cap;
+            for (int c = 0; c < n; ++c) out[count[wdw.most - (h1[c] - h0[c])]++] = c;
+        }
+    }
+    if (a->perm_elems < a->perm_h.size()) {
+        (void)hipFree(a->perm);
+        a->perm = nullptr;
+        a->perm_elems = 0;
+        WR_HIP(hipMalloc(&a->perm, a->perm_h.size() * sizeof(int)), "hipMalloc(adjoint order)");
+        a->perm_elems = a->perm_h.size();
+    }
+    WR_HIP(hipMemcpyAsync(a->perm, a->perm_h.data(), a->perm_h.size() * sizeof(int),
+                          hipMemcpyHostToDevice, ctx->stream),
+           "hipMemcpyAsync(adjoint order)");
+    // the recompute starts after the status word is cleared and the order is in
     WR_HIP(hipEventRecord(a->go, ctx->stream), "hipEventRecord");
     WR_HIP(hipStreamWaitEvent(a->steps_stream, a->go, 0), "hipStreamWaitEvent");
     for (size_t i = 0; i < wins.size(); ++i) {
@@ -1048,13 +1091,14 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
         // buffer b is free once the reverse sweep of window i - 2 has read it
         if (i >= 2) WR_HIP(hipStreamWaitEvent(a->steps_stream, a->freed[b], 0), "hipStreamWaitEvent");
         const int nA = n * (wdw.s1 - wdw.s0);
+        const int* perm = a->perm + i * cap;
         hipLaunchKernelGGL(k_adj_steps, dim3(blocks_for(nA, kAdjBlock)), dim3(kAdjBlock), 0,
-                           a->steps_stream, d, buf[b], n, wdw.s0, wdw.s1, 0);
+                           a->steps_stream, d, buf[b], perm, n, wdw.s0, wdw.s1, 0);
         WR_HIP(hipGetLastError(), "k_adj_steps launch");
         WR_HIP(hipEventRecord(a->ready[b], a->steps_stream), "hipEventRecord");
         WR_HIP(hipStreamWaitEvent(ctx->stream, a->ready[b], 0), "hipStreamWaitEvent");
         hipLaunchKernelGGL(k_adj_reverse, dim3(blocks_for(n, lpw)), dim3(kAdjBlock), 0,
-                           ctx->stream, d, buf[b], d_lam, n, wdw.s0, wdw.s1,
+                           ctx->stream, d, buf[b], perm, d_lam, n, wdw.s0, wdw.s1,
                            wdw.s1 == a->nsteps ? 1 : 0, 0, lpw);
         WR_HIP(hipGetLastError(), "k_adj_reverse launch");
         WR_HIP(hipEventRecord(a->freed[b], ctx->stream), "hipEventRecord");
