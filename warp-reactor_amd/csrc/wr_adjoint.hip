@@ -31,6 +31,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 
 #include <vector>
@@ -985,7 +986,10 @@ int wr_adjoint_replay(wr_context* ctx, const double* s0, const double* multiplie
 // kMinWindows of them when there are that many steps, so that recomputing a
 // window's steps (k_adj_steps, second stream) overlaps the reverse sweep of
 // the window after it (k_adj_reverse, the context's stream).
-constexpr size_t kWindowBudgetBytes = size_t(12) << 30;
+// The budget for one buffer is an eighth of the free device memory at the
+// backward call, between 1 and 32 GiB: longer windows leave fewer kernel
+// tails (at 64^3, 12 GiB gives windows of 2-3 steps).
+constexpr size_t kWindowBudgetMin = size_t(1) << 30, kWindowBudgetMax = size_t(32) << 30;
 constexpr int kMinWindows = 4;
 
 int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* mbar) {
@@ -1007,6 +1011,9 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
     const int lpw = a->lanes_per_wave > 0 ? a->lanes_per_wave : kAdjBlock;
     // windows from the end: the longest ending at s1 that fits the budget (at
     // least one step), at most ceil(nsteps / kMinWindows) steps long
+    size_t free_b = 0, total_b = 0;
+    WR_HIP(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t budget = std::min(kWindowBudgetMax, std::max(kWindowBudgetMin, free_b / 8));
     struct Window {
         int s0, s1, most;
     };
@@ -1026,7 +1033,7 @@ int wr_adjoint_backward(wr_context* ctx, const double* gN, double* g0, double* m
         int s0 = s1 - 1;
         while (s0 > 0 &&
                static_cast<size_t>(most_of(s0 - 1)) * kSub * cap * sizeof(double) <=
-                   kWindowBudgetBytes &&
+                   budget &&
                s1 - (s0 - 1) <= span && (a->window_max <= 0 || s1 - (s0 - 1) <= a->window_max))
             --s0;
         const int most = most_of(s0);
