@@ -346,9 +346,11 @@ int wr_adjoint_backward_gen(wr_context *ctx, unsigned long long generation, cons
 int wr_adjoint_tape(const wr_context *ctx, int32_t *hoff, double *htape, double *dts,
                     double *ckpt);
 /* The backward pass takes the recorded steps in windows from the last one
- * backwards, each window's accepted steps stored at once (sized to a 24 GiB
- * budget); max_steps > 0 caps the steps per window (0: the budget alone).
- * Results do not depend on it. */
+ * backwards, each window's accepted steps stored at once in one of two
+ * buffers (each sized to an eighth of the free device memory at the call,
+ * 1-32 GiB, and windows to at most a quarter of the steps); max_steps > 0
+ * caps the steps per window (0: the budget alone).  Results do not depend on
+ * it. */
 int wr_adjoint_set_window(wr_context *ctx, int max_steps);
 /* (new) Whether subsequent wr_adjoint_record calls apply the density
  * perturbations (0: no, the default; 1: yes, as wr_run with perturb = 1 from
